@@ -1,0 +1,231 @@
+"""SSIMULACRA2 throughput on one MI355X: 240 synthetic 1080p YUV420 frame pairs.
+
+A step is one pass of the plain FP32 path over the 240 pairs (8-bit YUV 4:2:0
+BT.709 limited, translating sinusoid content, distorted by per-frame 8x8 DCT
+quantisation with q ~ U{4..32} and chroma re-subsampling). Inputs are resident
+in HBM before the timed region; the pairs are spread over --handlers handlers,
+one host thread each, every pair a blocking VshipAMD_ComputeSSIMU2Device call
+(the drop-in path with device-resident planes).
+
+Prints one JSON line (rank 0) with the throughput, the roofline of the
+dominant kernel (HIP-event timed on its own stream after the timed region),
+the PCIe-inclusive rate of Vship_ComputeSSIMU2 on pinned host planes, and the
+FP64 oracle's CPU baseline on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import ctypes
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from vship_amd import ssimu2 as S  # noqa: E402
+from vship_amd import synth  # noqa: E402
+
+W, H, PAIRS = 1920, 1080, 240
+HBM_PEAK_GBS = 8000.0
+VALU_FP32_PEAK_TFLOPS = 78.6  # non-packed FP32 FMA rate: packed FP32 is disabled in this build
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_frames(n: int, seed: int = 0):
+    seq = synth.YuvSequence(W, H, n, bits=8, seed=seed)
+    with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        frames = list(ex.map(seq.frame, range(n)))
+    return frames
+
+
+def upload(frames):
+    """One device buffer holding every plane; returns (buf, ptrs[pair][img][3], strides)."""
+    sizes = [p.nbytes for p in frames[0][0]]
+    per_img = sum(sizes)
+    buf = S.DeviceBuffer(per_img * 2 * len(frames))
+    ptrs = []
+    off = 0
+    for ref, dist, _ in frames:
+        pair = []
+        for img in (ref, dist):
+            planes = []
+            for p in img:
+                buf.upload(p, off)
+                planes.append(buf.address + off)
+                off += p.nbytes
+            pair.append(planes)
+        ptrs.append(pair)
+    strides = [int(p.strides[0]) for p in frames[0][0]]
+    return buf, ptrs, strides
+
+
+def run_plain(handlers, ptrs, strides, scores):
+    n = len(handlers)
+    errors = []
+
+    def worker(k):
+        try:
+            h = handlers[k]
+            for i in range(k, len(ptrs), n):
+                scores[i] = h.compute_device(ptrs[i][0], ptrs[i][1], strides, strides)
+        except Exception as e:  # surfaced after join
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(n)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if errors:
+        raise errors[0]
+
+
+def cpu_baseline(frames, threads: int, pairs: int):
+    import oracle
+    cs = S.colorspace(W, H)
+    sample = [frames[i % len(frames)] for i in range(pairs)]
+
+    def one(fr):
+        ref, dist, _ = fr
+        st = S.host_strides(ref)
+        return oracle.ssimu2(cs, ref, st, cs, dist, st)[0]
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        vals = list(ex.map(one, sample))
+    dt = time.perf_counter() - t0
+    return pairs / dt, vals
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--handlers", type=int, default=4)
+    ap.add_argument("--cpu-pairs", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; frame pairs shard across GPUs with one "
+                                   "handler per device and no collectives, see DESIGN.md"}))
+        sys.exit(2)
+
+    t0 = time.perf_counter()
+    frames = make_frames(PAIRS)
+    log(f"generated {PAIRS} pairs in {time.perf_counter() - t0:.1f}s")
+    buf, ptrs, strides = upload(frames)
+    cs = S.colorspace(W, H)
+    handlers = [S.SSIMU2Handler(cs, cs) for _ in range(args.handlers)]
+    scores = [0.0] * PAIRS
+
+    for _ in range(args.warmup):
+        run_plain(handlers, ptrs, strides, scores)
+    lib = S.load_library()
+    S.check(lib.VshipAMD_DeviceSynchronize())
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run_plain(handlers, ptrs, strides, scores)
+    S.check(lib.VshipAMD_DeviceSynchronize())
+    elapsed = time.perf_counter() - t0
+    value = PAIRS * args.steps / elapsed
+    log(f"plain fp32: {value:.1f} pairs/s, mean score {np.mean(scores):.4f}")
+
+    # per-kernel device time of one pair, HIP events on the handler's stream
+    h0 = handlers[0]
+    h0.compute_device(ptrs[0][0], ptrs[0][1], strides, strides)
+    ms_front, ms_blur, ms_score = h0.profile(50)
+    bytes_pair, flops_pair = h0.work()
+    blur_bytes = 2.0 * 4.0 * 3491280.0  # both images' computed-map pixels read once, fp32
+    dominant = "blur" if ms_blur >= ms_front else "frontend"
+    blur_gbs = blur_bytes / (ms_blur * 1e-3) / 1e9
+    roofline = {
+        "kernel": "blur_map_kernel<float>", "bound": "hbm", "achieved": round(blur_gbs, 1),
+        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(blur_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+        "valu": {"achieved": round(flops_pair / (ms_blur * 1e-3) / 1e12, 2), "peak": VALU_FP32_PEAK_TFLOPS,
+                 "unit": "TFLOP/s", "frac": round(flops_pair / (ms_blur * 1e-3) / 1e12 / VALU_FP32_PEAK_TFLOPS, 4)},
+        "kernel_ms": {"frontend": round(ms_front, 4), "blur": round(ms_blur, 4), "score": round(ms_score, 4)},
+        "dominant": dominant,
+        "path_bytes_per_pair": bytes_pair,
+    }
+    traffic_file = os.path.join(ROOT, "profiles", "blur_map_1080p_traffic.json")
+    if os.path.exists(traffic_file):
+        roofline["traffic"] = json.load(open(traffic_file)).get("bytes_per_launch")
+
+    # PCIe-inclusive rate of the reference-shaped call on pinned host planes
+    pinned = []
+    try:
+        host_pairs = []
+        for ref, dist, _ in frames[:16]:
+            pair = []
+            for img in (ref, dist):
+                planes = []
+                for p in img:
+                    ptr = ctypes.c_void_p()
+                    S.check(lib.Vship_PinnedMalloc(ctypes.byref(ptr), p.nbytes))
+                    pinned.append(ptr)
+                    ctypes.memmove(ptr, p.ctypes.data, p.nbytes)
+                    planes.append(ptr.value)
+                pair.append(planes)
+            host_pairs.append(pair)
+
+        def host_worker(k, out):
+            h = handlers[k]
+            lib_ = S.load_library()
+            for i in range(k, 64, len(handlers)):
+                pr = host_pairs[i % len(host_pairs)]
+                p1, l1 = S._plane_args(pr[0], strides)
+                p2, l2 = S._plane_args(pr[1], strides)
+                sc = ctypes.c_double()
+                S.check(lib_.Vship_ComputeSSIMU2(h.handle, ctypes.byref(sc), p1, p2, l1, l2))
+        t0 = time.perf_counter()
+        ths = [threading.Thread(target=host_worker, args=(k, None)) for k in range(len(handlers))]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        host_rate = 64 / (time.perf_counter() - t0)
+    finally:
+        for p in pinned:
+            lib.Vship_PinnedFree(p)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        rate, vals = cpu_baseline(frames, threads, args.cpu_pairs)
+        err = max(abs(a - scores[i % PAIRS]) for i, a in enumerate(vals))
+        cpu = {"value": round(rate, 3), "unit": "pairs/s", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_pairs} of the 240 1080p pairs, FP64 oracle, one pair per thread",
+               "max_abs_delta_vs_gpu": round(err, 6)}
+
+    for h in handlers:
+        h.close()
+    buf.free()
+    out = {
+        "metric": "SSIMU2 frame-pairs/s at 1080p (plain FP32 drop-in path); % HBM peak",
+        "value": round(value, 2), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "240 pairs 1920x1080 YUV420 8-bit BT.709 limited, DCT-quantised q~U{4..32}, "
+                               "plain FP32 path, device-resident planes", "pairs_per_step": PAIRS,
+                   "handlers": args.handlers},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "pcie_inclusive_pairs_per_s": round(host_rate, 2),
+        "mean_score": round(float(np.mean(scores)), 5),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

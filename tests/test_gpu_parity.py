@@ -1,0 +1,185 @@
+"""GPU parity of the HIP path (through the C ABI) against the FP64 oracle and,
+where it was built, the reference libvship on the same inputs.
+
+Bars: |delta score| <= 0.005 against the oracle and the reference, identical
+inputs give exactly 100, stored XYB planes within 5e-5 of the oracle's.
+"""
+import zlib
+
+import numpy as np
+import pytest
+
+import oracle
+from vship_amd import ssimu2 as S
+from vship_amd import synth
+from tests import cases
+from tests import refvship
+
+pytestmark = pytest.mark.gpu
+
+SCORE_TOL = 0.005
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _oracle_score(cs1, p1, s1, cs2, p2, s2):
+    return oracle.ssimu2(cs1, p1, s1, cs2, p2, s2)
+
+
+@pytest.mark.parametrize("name,kw", cases.COLOR_CASES, ids=[c[0] for c in cases.COLOR_CASES])
+def test_colorspaces_vs_oracle(name, kw):
+    w, h = 131, 75
+    cs = S.colorspace(w, h, **kw)
+    ref, dist, strides = cases.perturbed_pair(cs, seed=zlib.crc32(name.encode()) % 1000, pad=7)
+    _say("case", name, w, h)
+    with S.SSIMU2Handler(cs, cs) as hd:
+        got = hd.compute(ref, dist, strides, strides)
+        feats = hd.features()
+        ident = hd.compute(ref, ref, strides, strides)
+    want, wf = _oracle_score(cs, ref, strides, cs, dist, strides)
+    assert abs(got - want) <= SCORE_TOL, (got, want)
+    np.testing.assert_allclose(feats, wf, rtol=2e-3, atol=2e-5)
+    assert ident == 100.0
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (3, 2), (8, 8), (17, 9), (64, 64), (65, 33), (135, 67), (300, 200)])
+def test_sizes_vs_oracle(w, h):
+    cs = S.colorspace(w, h)
+    ref, dist, strides = cases.perturbed_pair(cs, seed=w * 1000 + h)
+    _say("size", w, h)
+    with S.SSIMU2Handler(cs, cs) as hd:
+        got = hd.compute(ref, dist, strides, strides)
+        feats = hd.features()
+    want, wf = _oracle_score(cs, ref, strides, cs, dist, strides)
+    assert abs(got - want) <= SCORE_TOL, (got, want)
+    np.testing.assert_allclose(feats, wf, rtol=2e-3, atol=2e-5)
+
+
+@pytest.mark.parametrize("name,kw", [cases.COLOR_CASES[0], cases.COLOR_CASES[5], cases.COLOR_CASES[17]],
+                         ids=["yuv420", "yuv422", "rgb"])
+def test_xyb_planes_vs_oracle(name, kw):
+    w, h = 201, 117
+    cs = S.colorspace(w, h, **kw)
+    ref, dist, strides = cases.perturbed_pair(cs, seed=7)
+    _say("planes", name)
+    rgb = oracle.linear_rgb(cs, ref, strides)
+    levels = oracle.xyb_pyramid(rgb)
+    with S.SSIMU2Handler(cs, cs) as hd:
+        hd.compute(ref, dist, strides, strides)
+        checked = 0
+        for scale in range(5):
+            for plane in range(3):
+                try:
+                    got = hd.debug_plane(0, plane, scale)
+                except S.VshipError:
+                    continue
+                want = levels[scale][plane]
+                assert got.shape == want.shape
+                err = np.abs(got.astype(np.float64) - want).max()
+                assert err <= 5e-5, (scale, plane, err)
+                checked += 1
+    assert checked == 11
+
+
+def test_srgb_256_pair():
+    ref, dist = synth.srgb_pair(256, 256, seed=0)
+    cs = S.srgb_rgb_full(256, 256)
+    with S.SSIMU2Handler(cs, cs) as hd:
+        got = hd.compute(ref, dist)
+        assert hd.compute(ref, ref) == 100.0
+    want, _ = _oracle_score(cs, ref, [256] * 3, cs, dist, [256] * 3)
+    _say("srgb256", got, want)
+    assert abs(got - want) <= SCORE_TOL
+
+
+def test_1080p_yuv420_frames():
+    seq = synth.YuvSequence(1920, 1080, frames=3, seed=3)
+    cs = S.colorspace(1920, 1080)
+    with S.SSIMU2Handler(cs, cs) as hd:
+        for i in (0, 2):
+            ref, dist, q = seq.frame(i)
+            st = S.host_strides(ref)
+            got = hd.compute(ref, dist, st, st)
+            want, _ = _oracle_score(cs, ref, st, cs, dist, st)
+            _say("1080p frame", i, q, got, want)
+            assert abs(got - want) <= SCORE_TOL
+
+
+def test_strides_do_not_change_the_score():
+    cs = S.colorspace(97, 61)
+    ref, dist, strides = cases.perturbed_pair(cs, seed=11, pad=0)
+    ref_p, dist_p, strides_p = cases.perturbed_pair(cs, seed=11, pad=29)
+    with S.SSIMU2Handler(cs, cs) as hd:
+        a = hd.compute(ref, dist, strides, strides)
+        b = hd.compute(ref_p, dist_p, strides_p, strides_p)
+    assert a == b
+
+
+def test_device_entry_matches_host_entry():
+    cs = S.colorspace(160, 90)
+    ref, dist, strides = cases.perturbed_pair(cs, seed=5)
+    bufs = []
+    dptr = [[], []]
+    for k, planes in enumerate((ref, dist)):
+        for p in planes:
+            b = S.DeviceBuffer(p.nbytes)
+            b.upload(p)
+            bufs.append(b)
+            dptr[k].append(b.address)
+    with S.SSIMU2Handler(cs, cs) as hd:
+        host = hd.compute(ref, dist, strides, strides)
+        dev = hd.compute_device(dptr[0], dptr[1], strides, strides)
+    for b in bufs:
+        b.free()
+    assert host == dev
+
+
+def test_mixed_colourspaces_in_one_pair():
+    w, h = 120, 80
+    cs1 = S.colorspace(w, h)
+    cs2 = S.colorspace(w, h, sample=S.SAMPLE_UINT10, subw=0, subh=0)
+    r1, _, s1 = cases.perturbed_pair(cs1, seed=1)
+    r2, _, s2 = cases.perturbed_pair(cs2, seed=2)
+    with S.SSIMU2Handler(cs1, cs2) as hd:
+        got = hd.compute(r1, r2, s1, s2)
+    want, _ = _oracle_score(cs1, r1, s1, cs2, r2, s2)
+    assert abs(got - want) <= SCORE_TOL
+
+
+def test_error_paths_on_gpu():
+    with pytest.raises(S.VshipError) as e:
+        S.SSIMU2Handler(S.colorspace(64, 64), S.colorspace(64, 65))
+    assert e.value.code == 4  # DifferingInputType
+    cs = S.colorspace(32, 32)
+    hd = S.SSIMU2Handler(cs, cs)
+    hd.close()
+    ref, dist, st = cases.perturbed_pair(cs, seed=0)
+    with pytest.raises(S.VshipError) as e:
+        hd.compute(ref, dist, st, st)
+    assert e.value.code == 10  # BadHandler after free
+    assert S.device_count() >= 1
+    assert S.load_library().Vship_GPUFullCheck(0) == 0
+
+
+@pytest.mark.skipif(not refvship.available(), reason="reference libvship not built (oracle/_ref)")
+def test_against_reference_libvship():
+    ref_lib = refvship.RefVship()
+    checks = []
+    ref, dist = synth.srgb_pair(256, 256, seed=0)
+    cs = S.srgb_rgb_full(256, 256)
+    checks.append((cs, ref, [256] * 3, dist))
+    seq = synth.YuvSequence(1920, 1080, frames=2, seed=3)
+    r, d, _ = seq.frame(1)
+    checks.append((S.colorspace(1920, 1080), r, S.host_strides(r), d))
+    for name, kw in cases.COLOR_CASES[:10]:
+        c = S.colorspace(131, 75, **kw)
+        rr, dd, st = cases.perturbed_pair(c, seed=3)
+        checks.append((c, rr, st, dd))
+    for c, rr, st, dd in checks:
+        want = ref_lib.score(c, c, rr, st, dd, st)
+        with S.SSIMU2Handler(c, c) as hd:
+            got = hd.compute(rr, dd, st, st)
+        _say("ref", c.width, c.height, got, want)
+        assert abs(got - want) <= SCORE_TOL, (got, want)

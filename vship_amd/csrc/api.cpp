@@ -1,0 +1,479 @@
+// api.cpp - the C ABI (include/VshipAPI.h, include/vship_amd.h).
+//
+// Behaviour follows the reference's src/VshipLib.cpp:70-563: every entry point
+// returns a Vship_Exception, records the detailed message of a failure
+// globally and per handler, and never lets a C++ exception escape. Handler ids
+// index a table guarded by a mutex; unlike the reference (VshipLib.cpp:242-256)
+// a freed id is marked empty, so using it again returns Vship_BadHandler
+// instead of touching freed memory.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/VshipAPI.h"
+#include "../../include/vship_amd.h"
+#include "engine.hpp"
+#include "errors.hpp"
+
+namespace vship {
+
+const char* error_text(Vship_Exception code)
+{
+    switch (code) {
+    case Vship_NoError: return "NoError: the call succeeded.";
+    case Vship_OutOfVRAM: return "OutOfVRAM: a GPU memory allocation failed. (Advice) use fewer concurrent handlers.";
+    case Vship_OutOfRAM: return "OutOfRAM: a host memory allocation failed. (Advice) check the process memory usage.";
+    case Vship_HIPError: return "InternalError: a HIP runtime call failed inside the library.";
+    case Vship_BadDisplayModel: return "BadDisplayModel: unknown display model.";
+    case Vship_DifferingInputType: return "DifferingInputType: the two inputs do not have the same final width and height.";
+    case Vship_NonRGBSInput: return "NonRGBSInput: the input colourspace cannot be converted to linear RGB by this library.";
+    case Vship_BadPath: return "BadPath: a file could not be opened.";
+    case Vship_BadJson: return "BadJson: a JSON document could not be parsed.";
+    case Vship_DeviceCountError: return "DeviceCountError: the number of GPUs could not be queried. (Advice) check that the ROCm driver is loaded and the user is in the video and render groups.";
+    case Vship_NoDeviceDetected: return "NoDeviceDetected: no GPU was found. (Advice) check that the ROCm driver is loaded and the user is in the video and render groups.";
+    case Vship_BadDeviceArgument: return "BadDeviceArgument: gpu_id is negative or not below the number of GPUs.";
+    case Vship_BadDeviceCode: return "BadDeviceCode: a probe kernel did not run. (Advice) this library is built for gfx950 (MI355X) only.";
+    case Vship_BadHandler: return "BadHandler: the handler was never initialised or was already freed.";
+    case Vship_BadPointer: return "BadPointer: a pointer or size argument is not valid.";
+    case Vship_BadErrorType: return "BadErrorType: unknown error.";
+    }
+    return "BadErrorType: unknown error.";
+}
+
+std::string Error::message() const
+{
+    std::ostringstream ss;
+    ss << "VshipException" << '\n' << error_text(type_) << '\n' << " - At line " << line_ << " of " << file_ << '\n';
+    if (!detail_.empty()) ss << "Detail: " << detail_ << '\n';
+    return ss.str();
+}
+
+namespace {
+
+struct HandlerSlot {
+    std::unique_ptr<Ssimu2Handler> impl;
+    std::string last_error;
+    bool live = false;
+};
+
+std::mutex g_lock;
+std::vector<HandlerSlot> g_handlers;
+std::string g_last_error = Error(Vship_NoError, __FILE__, __LINE__).message();
+int g_device = 0;
+
+void set_last(const std::string& msg)
+{
+    std::lock_guard<std::mutex> g(g_lock);
+    g_last_error = msg;
+}
+
+int copy_message(const std::string& s, char* out, int len)
+{
+    if (len <= 0 || out == nullptr) return (int)s.size() + 1;
+    const int n = std::min(len - 1, (int)s.size());
+    std::memcpy(out, s.data(), (size_t)n);
+    out[n] = '\0';
+    return (int)s.size() + 1;
+}
+
+int device_count()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) VSHIP_THROW(Vship_DeviceCountError, "");
+    if (n == 0) VSHIP_THROW(Vship_NoDeviceDetected, "");
+    return n;
+}
+
+__global__ void probe_kernel(int* p) { p[0] = 4320984; }
+
+// Looks a live handler up; throws BadHandler otherwise.
+Ssimu2Handler* lookup(unsigned id, const char* where)
+{
+    std::lock_guard<std::mutex> g(g_lock);
+    if (id >= g_handlers.size() || !g_handlers[id].live)
+        throw Error(Vship_BadHandler, __FILE__, __LINE__, std::string("invalid handler id in ") + where);
+    return g_handlers[id].impl.get();
+}
+
+void record_handler_error(unsigned id, const std::string& msg)
+{
+    std::lock_guard<std::mutex> g(g_lock);
+    g_last_error = msg;
+    if (id < g_handlers.size() && g_handlers[id].live) g_handlers[id].last_error = msg;
+}
+
+template <typename F>
+Vship_Exception guarded(F&& f)
+{
+    try {
+        f();
+    } catch (const Error& e) {
+        set_last(e.message());
+        return e.type();
+    } catch (const std::bad_alloc&) {
+        set_last(Error(Vship_OutOfRAM, __FILE__, __LINE__).message());
+        return Vship_OutOfRAM;
+    } catch (...) {
+        set_last(Error(Vship_BadErrorType, __FILE__, __LINE__).message());
+        return Vship_BadErrorType;
+    }
+    return Vship_NoError;
+}
+
+template <typename F>
+Vship_Exception guarded_handler(unsigned id, F&& f)
+{
+    try {
+        f();
+    } catch (const Error& e) {
+        record_handler_error(id, e.message());
+        return e.type();
+    } catch (const std::bad_alloc&) {
+        record_handler_error(id, Error(Vship_OutOfRAM, __FILE__, __LINE__).message());
+        return Vship_OutOfRAM;
+    } catch (...) {
+        record_handler_error(id, Error(Vship_BadErrorType, __FILE__, __LINE__).message());
+        return Vship_BadErrorType;
+    }
+    return Vship_NoError;
+}
+
+Vship_Exception not_in_library(const char* what)
+{
+    set_last(Error(Vship_BadErrorType, __FILE__, __LINE__,
+                   std::string(what) + " is not part of this library (SSIMULACRA2 only)").message());
+    return Vship_BadErrorType;
+}
+
+}  // namespace
+}  // namespace vship
+
+using namespace vship;
+
+extern "C" {
+
+Vship_Version Vship_GetVersion(void)
+{
+    Vship_Version v;
+    v.major = 4;
+    v.minor = 1;
+    v.minorMinor = 0;
+    v.backend = Vship_HIP;
+    return v;
+}
+
+Vship_Exception Vship_GetDeviceCount(int* number)
+{
+    if (!number) return Vship_BadPointer;
+    return guarded([&] { *number = device_count(); });
+}
+
+Vship_Exception Vship_SetDevice(int gpu_id)
+{
+    return guarded([&] {
+        const int n = device_count();
+        if (gpu_id < 0 || gpu_id >= n) VSHIP_THROW(Vship_BadDeviceArgument, "In Vship_SetDevice");
+        std::lock_guard<std::mutex> g(g_lock);
+        g_device = gpu_id;
+    });
+}
+
+Vship_Exception Vship_GetDeviceInfo(Vship_DeviceInfo* info, int gpu_id)
+{
+    if (!info) return Vship_BadPointer;
+    return guarded([&] {
+        const int n = device_count();
+        if (gpu_id < 0 || gpu_id >= n) VSHIP_THROW(Vship_BadDeviceArgument, "while using Vship_GetDeviceInfo");
+        hipDeviceProp_t prop;
+        VSHIP_HIP_CHECK(hipGetDeviceProperties(&prop, gpu_id));
+        std::memcpy(info->name, prop.name, 256);
+        info->name[255] = '\0';
+        info->VRAMSize = prop.totalGlobalMem;
+        info->integrated = prop.integrated;
+        info->MultiProcessorCount = prop.multiProcessorCount;
+        info->WarpSize = prop.warpSize;
+    });
+}
+
+Vship_Exception Vship_GPUFullCheck(int gpu_id)
+{
+    return guarded([&] {
+        const int n = device_count();
+        if (gpu_id < 0 || gpu_id >= n) VSHIP_THROW(Vship_BadDeviceArgument, "");
+        VSHIP_HIP_CHECK(hipSetDevice(gpu_id));
+        int* d = nullptr;
+        int h = 0;
+        VSHIP_HIP_CHECK(hipMalloc((void**)&d, sizeof(int)));
+        VSHIP_HIP_CHECK(hipMemset(d, 0, sizeof(int)));
+        hipLaunchKernelGGL(probe_kernel, dim3(1), dim3(1), 0, 0, d);
+        const hipError_t le = hipGetLastError();
+        const hipError_t ce = hipMemcpy(&h, d, sizeof(int), hipMemcpyDeviceToHost);
+        (void)hipFree(d);
+        if (le != hipSuccess || ce != hipSuccess || h != 4320984) VSHIP_THROW(Vship_BadDeviceCode, "");
+    });
+}
+
+int Vship_GetErrorMessage(Vship_Exception exception, char* out_message, int len)
+{
+    return copy_message(error_text(exception), out_message, len);
+}
+
+int Vship_GetDetailedLastError(char* out_message, int len)
+{
+    std::string s;
+    {
+        std::lock_guard<std::mutex> g(g_lock);
+        s = g_last_error;
+    }
+    return copy_message(s, out_message, len);
+}
+
+Vship_Exception Vship_PinnedMalloc(void** ptr, uint64_t size)
+{
+    if (!ptr) return Vship_BadPointer;
+    if (hipHostMalloc(ptr, size, hipHostMallocDefault) != hipSuccess) {
+        set_last(Error(Vship_OutOfRAM, __FILE__, __LINE__, "In Vship_PinnedMalloc").message());
+        return Vship_OutOfRAM;
+    }
+    return Vship_NoError;
+}
+
+Vship_Exception Vship_PinnedFree(void* ptr)
+{
+    if (hipHostFree(ptr) != hipSuccess) {
+        set_last(Error(Vship_BadPointer, __FILE__, __LINE__, "hipHostFree failed on the pointer given to Vship_PinnedFree").message());
+        return Vship_BadPointer;
+    }
+    return Vship_NoError;
+}
+
+Vship_Exception Vship_SSIMU2Init2(Vship_SSIMU2Handler* handler, Vship_Colorspace_t src_colorspace,
+                                  Vship_Colorspace_t dis_colorspace, int gpu_id)
+{
+    if (!handler) return Vship_BadPointer;
+    std::unique_ptr<Ssimu2Handler> impl;
+    const Vship_Exception err = guarded([&] {
+        const int n = device_count();
+        if (gpu_id < 0 || gpu_id >= n) VSHIP_THROW(Vship_BadDeviceArgument, "In Vship_SSIMU2Init2");
+        impl.reset(new Ssimu2Handler(src_colorspace, dis_colorspace, gpu_id));
+    });
+    // like the reference, an id is handed out even when init fails, so that
+    // Vship_SSIMU2GetDetailedLastError can report it; compute on it fails.
+    std::lock_guard<std::mutex> g(g_lock);
+    unsigned id = 0;
+    while (id < g_handlers.size() && g_handlers[id].live) id++;
+    if (id == g_handlers.size()) g_handlers.emplace_back();
+    HandlerSlot& slot = g_handlers[id];
+    slot.impl = std::move(impl);
+    slot.live = true;
+    slot.last_error = (err == Vship_NoError) ? Error(Vship_NoError, __FILE__, __LINE__).message() : g_last_error;
+    handler->id = id;
+    return err;
+}
+
+Vship_Exception Vship_SSIMU2Init(Vship_SSIMU2Handler* handler, Vship_Colorspace_t src_colorspace,
+                                 Vship_Colorspace_t dis_colorspace)
+{
+    int dev;
+    {
+        std::lock_guard<std::mutex> g(g_lock);
+        dev = g_device;
+    }
+    return Vship_SSIMU2Init2(handler, src_colorspace, dis_colorspace, dev);
+}
+
+Vship_Exception Vship_SSIMU2Free(Vship_SSIMU2Handler handler)
+{
+    std::unique_ptr<Ssimu2Handler> impl;
+    {
+        std::lock_guard<std::mutex> g(g_lock);
+        if (handler.id >= g_handlers.size() || !g_handlers[handler.id].live) {
+            g_last_error = Error(Vship_BadHandler, __FILE__, __LINE__,
+                                 "Handler in Vship_SSIMU2Free is not valid: never allocated or already freed").message();
+            return Vship_BadHandler;
+        }
+        impl = std::move(g_handlers[handler.id].impl);
+        g_handlers[handler.id].live = false;
+        g_handlers[handler.id].last_error.clear();
+    }
+    return guarded([&] { impl.reset(); });
+}
+
+Vship_Exception Vship_ComputeSSIMU2(Vship_SSIMU2Handler handler, double* score, const uint8_t* srcp1[3],
+                                    const uint8_t* srcp2[3], const int64_t lineSize[3], const int64_t lineSize2[3])
+{
+    Ssimu2Handler* h = nullptr;
+    Vship_Exception e = guarded([&] { h = lookup(handler.id, "Vship_ComputeSSIMU2"); });
+    if (e != Vship_NoError) return e;
+    return guarded_handler(handler.id, [&] {
+        if (!h) VSHIP_THROW(Vship_BadHandler, "handler initialisation had failed");
+        if (!score || !srcp1 || !srcp2 || !lineSize || !lineSize2) VSHIP_THROW(Vship_BadPointer, "null argument");
+        *score = h->compute_host(srcp1, srcp2, lineSize, lineSize2);
+    });
+}
+
+int Vship_SSIMU2GetDetailedLastError(Vship_SSIMU2Handler handler, char* out_message, int len)
+{
+    std::string s;
+    {
+        std::lock_guard<std::mutex> g(g_lock);
+        if (handler.id >= g_handlers.size() || !g_handlers[handler.id].live) {
+            g_last_error = Error(Vship_BadHandler, __FILE__, __LINE__,
+                                 "Handler in Vship_SSIMU2GetDetailedLastError is not valid").message();
+            return 0;
+        }
+        s = g_handlers[handler.id].last_error;
+    }
+    return copy_message(s, out_message, len);
+}
+
+// ---- Butteraugli and CVVDP: declared for link compatibility only
+
+Vship_Exception Vship_ButteraugliInit(Vship_ButteraugliHandler*, Vship_Colorspace_t, Vship_Colorspace_t, int, float)
+{
+    return not_in_library("Butteraugli");
+}
+Vship_Exception Vship_ButteraugliInit2(Vship_ButteraugliHandler*, Vship_Colorspace_t, Vship_Colorspace_t, int, float, int)
+{
+    return not_in_library("Butteraugli");
+}
+Vship_Exception Vship_ButteraugliFree(Vship_ButteraugliHandler) { return not_in_library("Butteraugli"); }
+Vship_Exception Vship_ComputeButteraugli(Vship_ButteraugliHandler, Vship_ButteraugliScore*, const uint8_t*, int64_t,
+                                         const uint8_t*[3], const uint8_t*[3], const int64_t[3], const int64_t[3])
+{
+    return not_in_library("Butteraugli");
+}
+int Vship_ButteraugliGetDetailedLastError(Vship_ButteraugliHandler, char*, int) { return 0; }
+
+Vship_Exception Vship_CVVDPInit(Vship_CVVDPHandler*, Vship_Colorspace_t, Vship_Colorspace_t, float, bool, const char*)
+{
+    return not_in_library("CVVDP");
+}
+Vship_Exception Vship_CVVDPInit2(Vship_CVVDPHandler*, Vship_Colorspace_t, Vship_Colorspace_t, float, bool, const char*,
+                                 const char*)
+{
+    return not_in_library("CVVDP");
+}
+Vship_Exception Vship_CVVDPInit3(Vship_CVVDPHandler*, Vship_Colorspace_t, Vship_Colorspace_t, float, bool, const char*,
+                                 const char*, int)
+{
+    return not_in_library("CVVDP");
+}
+Vship_Exception Vship_CVVDPFree(Vship_CVVDPHandler) { return not_in_library("CVVDP"); }
+Vship_Exception Vship_CVVDPGetDistmapResolution(Vship_CVVDPHandler, int64_t*, int64_t*) { return not_in_library("CVVDP"); }
+Vship_Exception Vship_ResetCVVDP(Vship_CVVDPHandler) { return not_in_library("CVVDP"); }
+Vship_Exception Vship_ResetScoreCVVDP(Vship_CVVDPHandler) { return not_in_library("CVVDP"); }
+Vship_Exception Vship_LoadTemporalCVVDP(Vship_CVVDPHandler, const uint8_t*[3], const uint8_t*[3], const int64_t[3],
+                                        const int64_t[3])
+{
+    return not_in_library("CVVDP");
+}
+Vship_Exception Vship_ComputeCVVDP(Vship_CVVDPHandler, double*, const uint8_t*, int64_t, const uint8_t*[3],
+                                   const uint8_t*[3], const int64_t[3], const int64_t[3])
+{
+    return not_in_library("CVVDP");
+}
+int Vship_CVVDPGetDetailedLastError(Vship_CVVDPHandler, char*, int) { return 0; }
+
+// ---- extensions (include/vship_amd.h)
+
+Vship_Exception VshipAMD_ComputeSSIMU2Device(Vship_SSIMU2Handler handler, double* score, const uint8_t* dsrcp1[3],
+                                             const uint8_t* dsrcp2[3], const int64_t lineSize[3],
+                                             const int64_t lineSize2[3])
+{
+    Ssimu2Handler* h = nullptr;
+    Vship_Exception e = guarded([&] { h = lookup(handler.id, "VshipAMD_ComputeSSIMU2Device"); });
+    if (e != Vship_NoError) return e;
+    return guarded_handler(handler.id, [&] {
+        if (!h) VSHIP_THROW(Vship_BadHandler, "handler initialisation had failed");
+        if (!score || !dsrcp1 || !dsrcp2 || !lineSize || !lineSize2) VSHIP_THROW(Vship_BadPointer, "null argument");
+        *score = h->compute_device(dsrcp1, dsrcp2, lineSize, lineSize2);
+    });
+}
+
+Vship_Exception VshipAMD_SSIMU2Features(Vship_SSIMU2Handler handler, double features[108])
+{
+    Ssimu2Handler* h = nullptr;
+    Vship_Exception e = guarded([&] { h = lookup(handler.id, "VshipAMD_SSIMU2Features"); });
+    if (e != Vship_NoError) return e;
+    return guarded_handler(handler.id, [&] {
+        if (!h) VSHIP_THROW(Vship_BadHandler, "handler initialisation had failed");
+        if (!features) VSHIP_THROW(Vship_BadPointer, "null argument");
+        h->last_features(features);
+    });
+}
+
+Vship_Exception VshipAMD_SSIMU2DebugPlane(Vship_SSIMU2Handler handler, int image, int plane, int scale, float* out,
+                                          int64_t* width, int64_t* height)
+{
+    Ssimu2Handler* h = nullptr;
+    Vship_Exception e = guarded([&] { h = lookup(handler.id, "VshipAMD_SSIMU2DebugPlane"); });
+    if (e != Vship_NoError) return e;
+    return guarded_handler(handler.id, [&] {
+        if (!h) VSHIP_THROW(Vship_BadHandler, "handler initialisation had failed");
+        if (!width || !height) VSHIP_THROW(Vship_BadPointer, "null argument");
+        h->debug_plane(image, plane, scale, out, width, height);
+    });
+}
+
+Vship_Exception VshipAMD_SSIMU2Profile(Vship_SSIMU2Handler handler, int iters, double* ms_frontend, double* ms_blur,
+                                       double* ms_score)
+{
+    Ssimu2Handler* h = nullptr;
+    Vship_Exception e = guarded([&] { h = lookup(handler.id, "VshipAMD_SSIMU2Profile"); });
+    if (e != Vship_NoError) return e;
+    return guarded_handler(handler.id, [&] {
+        if (!h) VSHIP_THROW(Vship_BadHandler, "handler initialisation had failed");
+        if (iters <= 0 || !ms_frontend || !ms_blur || !ms_score) VSHIP_THROW(Vship_BadPointer, "bad argument");
+        h->profile(iters, ms_frontend, ms_blur, ms_score);
+    });
+}
+
+Vship_Exception VshipAMD_SSIMU2Work(Vship_SSIMU2Handler handler, double* bytes, double* flops)
+{
+    Ssimu2Handler* h = nullptr;
+    Vship_Exception e = guarded([&] { h = lookup(handler.id, "VshipAMD_SSIMU2Work"); });
+    if (e != Vship_NoError) return e;
+    return guarded_handler(handler.id, [&] {
+        if (!h) VSHIP_THROW(Vship_BadHandler, "handler initialisation had failed");
+        if (!bytes || !flops) VSHIP_THROW(Vship_BadPointer, "null argument");
+        h->work(bytes, flops);
+    });
+}
+
+Vship_Exception VshipAMD_DeviceMalloc(void** ptr, uint64_t size, int gpu_id)
+{
+    if (!ptr) return Vship_BadPointer;
+    return guarded([&] {
+        VSHIP_HIP_CHECK(hipSetDevice(gpu_id));
+        if (hipMalloc(ptr, size) != hipSuccess) VSHIP_THROW(Vship_OutOfVRAM, "In VshipAMD_DeviceMalloc");
+    });
+}
+
+Vship_Exception VshipAMD_DeviceFree(void* ptr)
+{
+    return guarded([&] { VSHIP_HIP_CHECK(hipFree(ptr)); });
+}
+
+Vship_Exception VshipAMD_MemcpyHtoD(void* dst, const void* src, uint64_t size)
+{
+    return guarded([&] { VSHIP_HIP_CHECK(hipMemcpy(dst, src, size, hipMemcpyHostToDevice)); });
+}
+
+Vship_Exception VshipAMD_MemcpyDtoH(void* dst, const void* src, uint64_t size)
+{
+    return guarded([&] { VSHIP_HIP_CHECK(hipMemcpy(dst, src, size, hipMemcpyDeviceToHost)); });
+}
+
+Vship_Exception VshipAMD_DeviceSynchronize(void)
+{
+    return guarded([&] { VSHIP_HIP_CHECK(hipDeviceSynchronize()); });
+}
+
+}  // extern "C"

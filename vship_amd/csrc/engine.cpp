@@ -1,0 +1,450 @@
+// engine.cpp - host side of the plain (one pair per call) SSIMULACRA2 path.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "errors.hpp"
+#include "kernels.hpp"
+#include "weights.hpp"
+
+namespace vship {
+
+namespace {
+
+const double kWeights[108] = VSHIP_SSIMU2_WEIGHTS;
+
+bool map_active(int plane, int scale, int map)
+{
+    const double w1 = kWeights[plane * 36 + scale * 6 + map];
+    const double w4 = kWeights[plane * 36 + scale * 6 + 3 + map];
+    return !(w1 <= kWeightPruning && w4 <= kWeightPruning);
+}
+
+int sample_bits(int s)
+{
+    switch (s) {
+    case Vship_SampleUINT8: return 8;
+    case Vship_SampleUINT9: return 9;
+    case Vship_SampleUINT10: return 10;
+    case Vship_SampleUINT12: return 12;
+    case Vship_SampleUINT14: return 14;
+    case Vship_SampleUINT16: return 16;
+    default: return 1;
+    }
+}
+
+// RGB -> XYZ per primaries set and XYZ -> BT.709; same constants as reference
+// src/gpuColorToLinear/primaries.hpp:17-131.
+const double kToXyz709[9] = {0.4124, 0.3576, 0.1805, 0.2126, 0.7152, 0.0722, 0.0193, 0.1192, 0.9505};
+const double kToXyz470M[9] = {0.6068, 0.1735, 0.2003, 0.2989, 0.5866, 0.1145, 0.0, 0.06609, 1.116};
+const double kToXyz470BG[9] = {0.4305, 0.3416, 0.1783, 0.2220, 0.7067, 0.07132, 0.0202, 0.1296, 0.9391};
+const double kToXyz170M[9] = {0.39349893, 0.3652766, 0.19162447, 0.2123645, 0.70109542, 0.08654008,
+                              0.01873804, 0.1119396, 0.95812235};
+const double kFromXyz709[9] = {3.2410, -1.5374, -0.4986, -0.9692, 1.8760, 0.0416, 0.0556, -0.2040, 1.0570};
+const double k2020To709[9] = {1.6605, -0.5876, -0.0728, -0.1246, 1.1329, -0.0083, -0.0182, -0.1006, 1.1187};
+const double kToXyz2020[9] = {0.6370, 0.1446, 0.1689, 0.2627, 0.6780, 0.0593, 0.0, 0.0281, 1.0610};
+
+void matmul3(const double a[9], const double b[9], double out[9])
+{
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) out[r * 3 + c] = a[r * 3 + 0] * b[0 * 3 + c] + a[r * 3 + 1] * b[1 * 3 + c] + a[r * 3 + 2] * b[2 * 3 + c];
+}
+
+// Source primaries -> BT.709 as one matrix (primaries.hpp:134-144,195-220).
+bool primaries_matrix(int prim, float out[9])
+{
+    const double* to = nullptr;
+    double m[9];
+    switch (prim) {
+    case Vship_PRIMARIES_BT709: return false;
+    case Vship_PRIMARIES_BT2020:
+        for (int i = 0; i < 9; i++) out[i] = (float)k2020To709[i];
+        return true;
+    case Vship_PRIMARIES_INTERNAL:
+        for (int i = 0; i < 9; i++) out[i] = (float)kFromXyz709[i];
+        return true;
+    case Vship_PRIMARIES_BT470_M: to = kToXyz470M; break;
+    case Vship_PRIMARIES_BT470_BG: to = kToXyz470BG; break;
+    case Vship_PRIMARIES_ST170_M:
+    case Vship_PRIMARIES_ST240_M: to = kToXyz170M; break;
+    default: return false;
+    }
+    (void)kToXyz709;
+    (void)kToXyz2020;
+    matmul3(kFromXyz709, to, m);
+    for (int i = 0; i < 9; i++) out[i] = (float)m[i];
+    return true;
+}
+
+int64_t half_up(int64_t n) { return (n - 1) / 2 + 1; }
+
+int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+}  // namespace
+
+ImageSpec make_image_spec(const Vship_Colorspace_t& cs)
+{
+    ImageSpec s;
+    std::memset(&s, 0, sizeof(s));
+    switch (cs.sample) {
+    case Vship_SampleFLOAT: case Vship_SampleHALF: case Vship_SampleUINT8: case Vship_SampleUINT9:
+    case Vship_SampleUINT10: case Vship_SampleUINT12: case Vship_SampleUINT14: case Vship_SampleUINT16: break;
+    default: VSHIP_THROW(Vship_NonRGBSInput, "unknown Vship_Sample_t value");
+    }
+    if (cs.range != Vship_RangeFull && cs.range != Vship_RangeLimited)
+        VSHIP_THROW(Vship_NonRGBSInput, "unknown Vship_Range_t value");
+    if (cs.width <= 0 || cs.height <= 0 || cs.width > (1 << 20) || cs.height > (1 << 20))
+        VSHIP_THROW(Vship_NonRGBSInput, "width and height must be in [1, 2^20]");
+    if (cs.subsampling.subw < 0 || cs.subsampling.subw > 2 || cs.subsampling.subh < 0 || cs.subsampling.subh > 1)
+        VSHIP_THROW(Vship_NonRGBSInput, "chroma subsampling supported: subw in [0, 2], subh in [0, 1]");
+    if (cs.chromaLocation < Vship_ChromaLoc_Left || cs.chromaLocation > Vship_ChromaLoc_Top)
+        VSHIP_THROW(Vship_NonRGBSInput, "unknown Vship_ChromaLocation_t value");
+    if (cs.colorFamily != Vship_ColorYUV && cs.colorFamily != Vship_ColorRGB)
+        VSHIP_THROW(Vship_NonRGBSInput, "unknown Vship_ColorFamily_t value");
+    switch (cs.YUVMatrix) {
+    case Vship_MATRIX_RGB: case Vship_MATRIX_BT709: case Vship_MATRIX_BT470_BG: case Vship_MATRIX_ST170_M:
+    case Vship_MATRIX_BT2020_NCL: case Vship_MATRIX_BT2020_CL: case Vship_MATRIX_BT2100_ICTCP: break;
+    default: VSHIP_THROW(Vship_NonRGBSInput, "unknown Vship_YUVMatrix_t value");
+    }
+    switch (cs.transferFunction) {
+    case Vship_TRC_BT709: case Vship_TRC_BT470_M: case Vship_TRC_BT470_BG: case Vship_TRC_BT601:
+    case Vship_TRC_ST240_M: case Vship_TRC_Linear: case Vship_TRC_sRGB: case Vship_TRC_PQ:
+    case Vship_TRC_ST428: case Vship_TRC_HLG: break;
+    default: VSHIP_THROW(Vship_NonRGBSInput, "unknown Vship_TransferFunction_t value");
+    }
+    switch (cs.primaries) {
+    case Vship_PRIMARIES_INTERNAL: case Vship_PRIMARIES_BT709: case Vship_PRIMARIES_BT470_M:
+    case Vship_PRIMARIES_BT470_BG: case Vship_PRIMARIES_ST170_M: case Vship_PRIMARIES_ST240_M:
+    case Vship_PRIMARIES_BT2020: break;
+    default: VSHIP_THROW(Vship_NonRGBSInput, "unknown Vship_Primaries_t value");
+    }
+    const int64_t tw = cs.target_width == -1 ? cs.width : cs.target_width;
+    const int64_t th = cs.target_height == -1 ? cs.height : cs.target_height;
+    const bool cropped = cs.crop.top || cs.crop.bottom || cs.crop.left || cs.crop.right;
+    if (cropped || tw != cs.width || th != cs.height)
+        VSHIP_THROW(Vship_NonRGBSInput, "crop and resize are not supported by this library");
+
+    FrontColor& fc = s.fc;
+    fc.sample = cs.sample == Vship_SampleUINT8 ? 0 : (cs.sample == Vship_SampleHALF ? 2 : (cs.sample == Vship_SampleFLOAT ? 3 : 1));
+    fc.bits = sample_bits(cs.sample);
+    fc.bytes = cs.sample == Vship_SampleUINT8 ? 1 : (cs.sample == Vship_SampleFLOAT ? 4 : 2);
+    fc.range = cs.range;
+    fc.yuv = cs.colorFamily == Vship_ColorYUV;
+    fc.subw = cs.subsampling.subw;
+    fc.subh = cs.subsampling.subh;
+    fc.h_cosited = (cs.chromaLocation == Vship_ChromaLoc_Left || cs.chromaLocation == Vship_ChromaLoc_TopLeft);
+    fc.v_cosited = (cs.chromaLocation == Vship_ChromaLoc_Top || cs.chromaLocation == Vship_ChromaLoc_TopLeft);
+    fc.matrix = cs.YUVMatrix;
+    fc.transfer = cs.transferFunction;
+    fc.use_prim = primaries_matrix(cs.primaries, fc.prim) ? 1 : 0;
+    s.width = cs.width;
+    s.height = cs.height;
+    s.plane_w[0] = cs.width;
+    s.plane_h[0] = cs.height;
+    for (int p = 1; p < 3; p++) {
+        s.plane_w[p] = ((cs.width - 1) >> fc.subw) + 1;
+        s.plane_h[p] = ((cs.height - 1) >> fc.subh) + 1;
+    }
+    return s;
+}
+
+BlurTaps make_blur_taps()
+{
+    BlurTaps t;
+    const float sigma = 1.5f;
+    const double tau = 6.28318530718;
+    t.integral[0] = 0.0f;
+    for (int i = 0; i < 17; i++) {
+        const float e = std::exp((float)(-(8 - i) * (8 - i)) / (2 * sigma * sigma));
+        t.g[i] = (float)((double)e / std::sqrt(tau * sigma * sigma));
+        t.integral[i + 1] = t.integral[i] + t.g[i];
+    }
+    return t;
+}
+
+PairLayout make_pair_layout(int w, int h)
+{
+    PairLayout L;
+    std::memset(&L.geom, 0, sizeof(L.geom));
+    int64_t cw = w, ch = h, off = 0;
+    for (int s = 0; s < kUsedScales; s++) {
+        L.geom.w[s] = (int32_t)cw;
+        L.geom.h[s] = (int32_t)ch;
+        L.geom.pitch[s] = (int32_t)round_up(cw, 64);
+        for (int p = 0; p < 3; p++) {
+            const bool any = map_active(p, s, 0) || map_active(p, s, 1) || map_active(p, s, 2);
+            if (any) {
+                L.geom.off[s][p] = off;
+                off += (int64_t)L.geom.pitch[s] * ch;
+            } else {
+                L.geom.off[s][p] = -1;
+            }
+        }
+        cw = half_up(cw);
+        ch = half_up(ch);
+    }
+    L.geom.image_elems = off;
+    L.front_tiles_x = (w + kFrontTile - 1) / kFrontTile;
+    L.front_tiles_per_image = L.front_tiles_x * ((h + kFrontTile - 1) / kFrontTile);
+    int tiles = 0;
+    for (int s = 0; s < kUsedScales; s++) {
+        for (int p = 0; p < 3; p++) {
+            if (L.geom.off[s][p] < 0) continue;
+            MapJob m;
+            std::memset(&m, 0, sizeof(m));
+            m.a = reinterpret_cast<const void*>(L.geom.off[s][p]);
+            m.b = reinterpret_cast<const void*>(L.geom.off[s][p]);
+            m.w = L.geom.w[s];
+            m.h = L.geom.h[s];
+            m.pitch = L.geom.pitch[s];
+            m.flags = (map_active(p, s, 0) ? kMapSsim : 0) | (map_active(p, s, 1) ? kMapArtifact : 0) |
+                      (map_active(p, s, 2) ? kMapDetail : 0);
+            m.tiles_x = (m.w + kBlurTW - 1) / kBlurTW;
+            m.ntiles = m.tiles_x * ((m.h + kBlurTH - 1) / kBlurTH);
+            m.tile_start = tiles;
+            m.pair = 0;
+            m.feature_base = p * 36 + s * 6;
+            tiles += m.ntiles;
+            L.maps.push_back(m);
+        }
+    }
+    L.blur_tiles = tiles;
+    return L;
+}
+
+Ssimu2Handler::Ssimu2Handler(const Vship_Colorspace_t& src, const Vship_Colorspace_t& dis, int device)
+    : device_(device)
+{
+    spec_[0] = make_image_spec(src);
+    spec_[1] = make_image_spec(dis);
+    if (spec_[0].width != spec_[1].width || spec_[0].height != spec_[1].height)
+        VSHIP_THROW(Vship_DifferingInputType, "reference and distorted sizes differ");
+    width_ = (int)spec_[0].width;
+    height_ = (int)spec_[0].height;
+    layout_ = make_pair_layout(width_, height_);
+    taps_ = make_blur_taps();
+    front_lds_ = std::max(frontend_lds_bytes(spec_[0].fc), frontend_lds_bytes(spec_[1].fc));
+
+    VSHIP_HIP_CHECK(hipSetDevice(device_));
+    VSHIP_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    auto dmalloc = [](void** p, size_t bytes) {
+        if (hipMalloc(p, bytes) != hipSuccess) VSHIP_THROW(Vship_OutOfVRAM, "hipMalloc failed in SSIMU2 init");
+    };
+    const size_t elems = (size_t)layout_.geom.image_elems;
+    for (int i = 0; i < 2; i++) dmalloc(&xyb_[i], elems * sizeof(float));
+    dmalloc((void**)&d_front_jobs_, 2 * sizeof(FrontJob));
+    dmalloc((void**)&d_colors_, 2 * sizeof(FrontColor));
+    dmalloc((void**)&d_geom_, sizeof(PairGeom));
+    dmalloc((void**)&d_maps_, layout_.maps.size() * sizeof(MapJob));
+    dmalloc((void**)&d_score_jobs_, sizeof(ScoreJob));
+    dmalloc((void**)&d_counts_, 4 * sizeof(int));
+    dmalloc((void**)&d_partials_, (size_t)layout_.blur_tiles * kPartialStride * sizeof(float));
+    dmalloc((void**)&d_features_, 108 * sizeof(double));
+    dmalloc((void**)&d_score_, sizeof(double));
+    if (hipHostMalloc((void**)&h_front_jobs_, 2 * sizeof(FrontJob), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&h_score_, sizeof(double), hipHostMallocDefault) != hipSuccess)
+        VSHIP_THROW(Vship_OutOfRAM, "hipHostMalloc failed in SSIMU2 init");
+
+    // static tables
+    std::vector<MapJob> maps = layout_.maps;
+    for (auto& m : maps) {
+        const int64_t off = reinterpret_cast<int64_t>(m.a);
+        m.a = static_cast<const float*>(xyb_[0]) + off;
+        m.b = static_cast<const float*>(xyb_[1]) + off;
+    }
+    FrontColor colors[2] = {spec_[0].fc, spec_[1].fc};
+    ScoreJob sj = {0, (int32_t)maps.size(), {0, 0}};
+    const int counts[4] = {2 * layout_.front_tiles_per_image, layout_.blur_tiles, 1, 0};
+    VSHIP_HIP_CHECK(hipMemcpy(d_maps_, maps.data(), maps.size() * sizeof(MapJob), hipMemcpyHostToDevice));
+    VSHIP_HIP_CHECK(hipMemcpy(d_colors_, colors, sizeof(colors), hipMemcpyHostToDevice));
+    VSHIP_HIP_CHECK(hipMemcpy(d_geom_, &layout_.geom, sizeof(PairGeom), hipMemcpyHostToDevice));
+    VSHIP_HIP_CHECK(hipMemcpy(d_score_jobs_, &sj, sizeof(sj), hipMemcpyHostToDevice));
+    VSHIP_HIP_CHECK(hipMemcpy(d_counts_, counts, sizeof(counts), hipMemcpyHostToDevice));
+    VSHIP_HIP_CHECK(hipMemset(d_features_, 0, 108 * sizeof(double)));
+    for (int i = 0; i < 2; i++) {
+        FrontJob& j = h_front_jobs_[i];
+        std::memset(&j, 0, sizeof(j));
+        j.out = xyb_[i];
+        j.width = width_;
+        j.height = height_;
+        j.tiles_x = layout_.front_tiles_x;
+        j.tile_start = i * layout_.front_tiles_per_image;
+        j.pair = 0;
+        j.color = i;
+    }
+}
+
+Ssimu2Handler::~Ssimu2Handler()
+{
+    (void)hipSetDevice(device_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (int i = 0; i < 2; i++) {
+        (void)hipFree(xyb_[i]);
+        for (int p = 0; p < 3; p++) (void)hipFree(staging_[i][p]);
+    }
+    (void)hipFree(d_front_jobs_);
+    (void)hipFree(d_colors_);
+    (void)hipFree(d_geom_);
+    (void)hipFree(d_maps_);
+    (void)hipFree(d_score_jobs_);
+    (void)hipFree(d_counts_);
+    (void)hipFree(d_partials_);
+    (void)hipFree(d_features_);
+    (void)hipFree(d_score_);
+    (void)hipHostFree(h_front_jobs_);
+    (void)hipHostFree(h_score_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Ssimu2Handler::ensure_staging(int image, const int64_t ls[3])
+{
+    const ImageSpec& s = spec_[image];
+    for (int p = 0; p < 3; p++) {
+        const int64_t row = s.plane_w[p] * s.fc.bytes;
+        if (ls[p] < row) VSHIP_THROW(Vship_BadPointer, "lineSize smaller than one row of samples");
+        const size_t need = (size_t)(ls[p] * s.plane_h[p]);
+        if (need > staging_cap_[image][p]) {
+            (void)hipFree(staging_[image][p]);
+            staging_[image][p] = nullptr;
+            staging_cap_[image][p] = 0;
+            if (hipMalloc((void**)&staging_[image][p], need) != hipSuccess)
+                VSHIP_THROW(Vship_OutOfVRAM, "hipMalloc of the input staging buffer failed");
+            staging_cap_[image][p] = need;
+        }
+    }
+}
+
+void Ssimu2Handler::launch_all()
+{
+    VSHIP_HIP_CHECK(hipMemcpyAsync(d_front_jobs_, h_front_jobs_, 2 * sizeof(FrontJob), hipMemcpyHostToDevice, stream_));
+    VSHIP_HIP_CHECK(launch_frontend<float>(d_front_jobs_, 2, d_counts_ + 0, 2 * layout_.front_tiles_per_image,
+                                           d_colors_, d_geom_, front_lds_, stream_));
+    VSHIP_HIP_CHECK(launch_blur<float>(d_maps_, (int)layout_.maps.size(), d_counts_ + 1, layout_.blur_tiles, taps_,
+                                       d_partials_, stream_));
+    VSHIP_HIP_CHECK(launch_score(d_score_jobs_, d_counts_ + 2, 1, d_maps_, d_partials_, d_features_, d_score_, stream_));
+}
+
+void Ssimu2Handler::run(const uint8_t* const p1[3], const uint8_t* const p2[3], const int64_t ls1[3],
+                        const int64_t ls2[3])
+{
+    for (int p = 0; p < 3; p++) {
+        h_front_jobs_[0].plane[p] = p1[p];
+        h_front_jobs_[0].stride[p] = ls1[p];
+        h_front_jobs_[1].plane[p] = p2[p];
+        h_front_jobs_[1].stride[p] = ls2[p];
+    }
+    launch_all();
+    VSHIP_HIP_CHECK(hipMemcpyAsync(h_score_, d_score_, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    VSHIP_HIP_CHECK(hipStreamSynchronize(stream_));
+    have_run_ = true;
+}
+
+double Ssimu2Handler::compute_host(const uint8_t* const src1[3], const uint8_t* const src2[3], const int64_t ls1[3],
+                                   const int64_t ls2[3])
+{
+    VSHIP_HIP_CHECK(hipSetDevice(device_));
+    const uint8_t* const* srcs[2] = {src1, src2};
+    const int64_t* lss[2] = {ls1, ls2};
+    const uint8_t* dev[2][3];
+    for (int i = 0; i < 2; i++) {
+        ensure_staging(i, lss[i]);
+        const ImageSpec& s = spec_[i];
+        for (int p = 0; p < 3; p++) {
+            if (srcs[i][p] == nullptr) VSHIP_THROW(Vship_BadPointer, "null plane pointer");
+            // same extent as the reference's upload (vshipColor.hpp:96)
+            const size_t bytes = (size_t)(lss[i][p] * s.plane_h[p] - (lss[i][p] - s.fc.bytes * s.plane_w[p]));
+            VSHIP_HIP_CHECK(hipMemcpyAsync(staging_[i][p], srcs[i][p], bytes, hipMemcpyHostToDevice, stream_));
+            dev[i][p] = staging_[i][p];
+        }
+    }
+    run(dev[0], dev[1], ls1, ls2);
+    return *h_score_;
+}
+
+double Ssimu2Handler::compute_device(const uint8_t* const src1[3], const uint8_t* const src2[3], const int64_t ls1[3],
+                                     const int64_t ls2[3])
+{
+    VSHIP_HIP_CHECK(hipSetDevice(device_));
+    for (int p = 0; p < 3; p++) {
+        if (src1[p] == nullptr || src2[p] == nullptr) VSHIP_THROW(Vship_BadPointer, "null plane pointer");
+        if (ls1[p] < spec_[0].plane_w[p] * spec_[0].fc.bytes || ls2[p] < spec_[1].plane_w[p] * spec_[1].fc.bytes)
+            VSHIP_THROW(Vship_BadPointer, "lineSize smaller than one row of samples");
+    }
+    run(src1, src2, ls1, ls2);
+    return *h_score_;
+}
+
+void Ssimu2Handler::last_features(double out[108])
+{
+    VSHIP_HIP_CHECK(hipSetDevice(device_));
+    VSHIP_HIP_CHECK(hipMemcpy(out, d_features_, 108 * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+void Ssimu2Handler::debug_plane(int image, int plane, int scale, float* out, int64_t* w, int64_t* h)
+{
+    if (image < 0 || image > 1 || plane < 0 || plane > 2 || scale < 0 || scale >= kUsedScales ||
+        layout_.geom.off[scale][plane] < 0)
+        VSHIP_THROW(Vship_BadPointer, "no such stored XYB plane");
+    *w = layout_.geom.w[scale];
+    *h = layout_.geom.h[scale];
+    if (!out) return;
+    VSHIP_HIP_CHECK(hipSetDevice(device_));
+    const float* src = static_cast<const float*>(xyb_[image]) + layout_.geom.off[scale][plane];
+    VSHIP_HIP_CHECK(hipMemcpy2D(out, *w * sizeof(float), src, layout_.geom.pitch[scale] * sizeof(float),
+                                *w * sizeof(float), *h, hipMemcpyDeviceToHost));
+}
+
+void Ssimu2Handler::profile(int iters, double* ms_front, double* ms_blur, double* ms_score)
+{
+    if (!have_run_) VSHIP_THROW(Vship_BadPointer, "profile needs one compute call first");
+    VSHIP_HIP_CHECK(hipSetDevice(device_));
+    hipEvent_t ev[4];
+    for (auto& e : ev) VSHIP_HIP_CHECK(hipEventCreate(&e));
+    double acc[3] = {0, 0, 0};
+    for (int it = 0; it < iters; it++) {
+        VSHIP_HIP_CHECK(hipEventRecord(ev[0], stream_));
+        VSHIP_HIP_CHECK(launch_frontend<float>(d_front_jobs_, 2, d_counts_ + 0, 2 * layout_.front_tiles_per_image,
+                                               d_colors_, d_geom_, front_lds_, stream_));
+        VSHIP_HIP_CHECK(hipEventRecord(ev[1], stream_));
+        VSHIP_HIP_CHECK(launch_blur<float>(d_maps_, (int)layout_.maps.size(), d_counts_ + 1, layout_.blur_tiles,
+                                           taps_, d_partials_, stream_));
+        VSHIP_HIP_CHECK(hipEventRecord(ev[2], stream_));
+        VSHIP_HIP_CHECK(launch_score(d_score_jobs_, d_counts_ + 2, 1, d_maps_, d_partials_, d_features_, d_score_,
+                                     stream_));
+        VSHIP_HIP_CHECK(hipEventRecord(ev[3], stream_));
+        VSHIP_HIP_CHECK(hipEventSynchronize(ev[3]));
+        for (int k = 0; k < 3; k++) {
+            float ms = 0.0f;
+            VSHIP_HIP_CHECK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+            acc[k] += ms;
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    *ms_front = acc[0] / iters;
+    *ms_blur = acc[1] / iters;
+    *ms_score = acc[2] / iters;
+}
+
+void Ssimu2Handler::work(double* bytes, double* flops) const
+{
+    double in_bytes = 0.0;
+    for (int i = 0; i < 2; i++)
+        for (int p = 0; p < 3; p++) in_bytes += (double)spec_[i].plane_w[p] * spec_[i].plane_h[p] * spec_[i].fc.bytes;
+    double active = 0.0, fl = 0.0, blocks = 0.0;
+    for (const auto& m : layout_.maps) {
+        const double px = (double)m.w * m.h;
+        active += px;
+        const bool ssim = m.flags & kMapSsim;
+        const bool edge = m.flags & (kMapArtifact | kMapDetail);
+        const int n_edge = ((m.flags & kMapArtifact) ? 1 : 0) + ((m.flags & kMapDetail) ? 1 : 0);
+        fl += px * (70.0 * (ssim ? 4 : 2) + 23.0 * ssim + 10.0 * edge + 4.0 * n_edge);
+        blocks += std::ceil(m.w / 16.0) * std::ceil(m.h / 16.0);
+    }
+    // 2 * in_bytes + 4 * B * A + 48 * nblocks with B = 4 (float planes)
+    *bytes = in_bytes + 4.0 * 4.0 * active + 48.0 * blocks;
+    *flops = fl;
+}
+
+}  // namespace vship

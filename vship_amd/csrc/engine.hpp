@@ -1,0 +1,102 @@
+// engine.hpp - host side of the SSIMULACRA2 path on one MI355X.
+//
+// Ssimu2Plan validates two Vship_Colorspace_t descriptors and derives the
+// pyramid geometry, the active (plane, scale) maps and the launch tables.
+// Ssimu2Handler owns one HIP stream and every device buffer one pair needs,
+// allocated once in the constructor (the reference allocates and frees
+// ~100 MB per call, src/ssimu2/main.hpp:126-152); compute() issues three
+// kernels (front-end, blur/maps, FP64 score) and one 8-byte read-back.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/VshipAPI.h"
+#include "ssimu2_common.hpp"
+
+namespace vship {
+
+// Validated per-image colour description.
+struct ImageSpec {
+    FrontColor fc;
+    int64_t width, height;        // input size (level 0 when no crop/resize)
+    int64_t plane_w[3], plane_h[3];
+};
+
+ImageSpec make_image_spec(const Vship_Colorspace_t& cs);
+
+// Blur taps computed like the reference's GaussianHandle::init
+// (src/ssimu2/gaussianblur.hpp:7-14): float values and float prefix sums.
+BlurTaps make_blur_taps();
+
+// Geometry and map jobs of one pair of size w x h; xyb offsets relative to the
+// start of each image's XYB buffer.
+struct PairLayout {
+    PairGeom geom;
+    int front_tiles_per_image;
+    int front_tiles_x;
+    std::vector<MapJob> maps;     // a/b filled with element offsets, not pointers
+    int blur_tiles;
+};
+
+PairLayout make_pair_layout(int w, int h);
+
+class Ssimu2Handler {
+public:
+    Ssimu2Handler(const Vship_Colorspace_t& src, const Vship_Colorspace_t& dis, int device);
+    ~Ssimu2Handler();
+    Ssimu2Handler(const Ssimu2Handler&) = delete;
+    Ssimu2Handler& operator=(const Ssimu2Handler&) = delete;
+
+    int device() const { return device_; }
+
+    // Host planes (reference semantics): upload, run, wait, return the score.
+    double compute_host(const uint8_t* const src1[3], const uint8_t* const src2[3], const int64_t ls1[3],
+                        const int64_t ls2[3]);
+    // Planes already in device memory of this handler's GPU.
+    double compute_device(const uint8_t* const src1[3], const uint8_t* const src2[3], const int64_t ls1[3],
+                          const int64_t ls2[3]);
+
+    // Features (108 doubles) of the last compute.
+    void last_features(double out[108]);
+    // Dense copy of one XYB plane of the last compute (image 0 or 1).
+    void debug_plane(int image, int plane, int scale, float* out, int64_t* w, int64_t* h);
+    // Average device time (ms) of each kernel over `iters` replays of the last
+    // compute, measured with HIP events on the handler's stream.
+    void profile(int iters, double* ms_front, double* ms_blur, double* ms_score);
+    // Algorithmic work of one pair (bytes and blur-stage FLOPs).
+    void work(double* bytes, double* flops) const;
+
+private:
+    void run(const uint8_t* const p1[3], const uint8_t* const p2[3], const int64_t ls1[3], const int64_t ls2[3]);
+    void launch_all();
+    void ensure_staging(int image, const int64_t ls[3]);
+
+    int device_;
+    hipStream_t stream_ = nullptr;
+    ImageSpec spec_[2];
+    PairLayout layout_;
+    BlurTaps taps_;
+    int width_, height_;
+    size_t front_lds_ = 0;
+
+    void* xyb_[2] = {nullptr, nullptr};       // float planes per image
+    FrontJob* d_front_jobs_ = nullptr;
+    FrontColor* d_colors_ = nullptr;
+    PairGeom* d_geom_ = nullptr;
+    MapJob* d_maps_ = nullptr;
+    ScoreJob* d_score_jobs_ = nullptr;
+    int* d_counts_ = nullptr;                  // {front tiles, blur tiles, pairs}
+    float* d_partials_ = nullptr;
+    double* d_features_ = nullptr;
+    double* d_score_ = nullptr;
+    FrontJob* h_front_jobs_ = nullptr;         // pinned
+    double* h_score_ = nullptr;                // pinned
+    uint8_t* staging_[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    size_t staging_cap_[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    bool have_run_ = false;
+};
+
+}  // namespace vship

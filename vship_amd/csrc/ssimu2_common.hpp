@@ -1,0 +1,109 @@
+// ssimu2_common.hpp - types shared by the host engine and the gfx950 kernels of
+// the SSIMULACRA2 path.
+//
+// Data layout in HBM (per image of a pair): only the 11 (plane, scale) XYB
+// planes the score uses are stored, each a dense row-major W_s x H_s array
+// with a row pitch rounded up to 64 elements, back to back in one buffer.
+// The linear-RGB pyramid of the reference (src/ssimu2/main.hpp:30-52) never
+// reaches HBM: the front-end kernel builds levels 0..4 of a 64x64 level-0 tile
+// in registers/LDS and writes XYB directly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+namespace vship {
+
+constexpr int kScales = 6;        // pyramid levels of the reference (main.hpp:18)
+constexpr int kUsedScales = 5;    // scale 5 carries no active map
+constexpr int kMaxJobsPerPair = 11;
+constexpr float kC2 = 0.0009f;    // SSIM constant (score.hpp:272-273)
+
+// Front-end tile: 64x64 level-0 pixels, 256 threads, each owning a 4x4 block.
+constexpr int kFrontTile = 64;
+constexpr int kFrontThreads = 256;
+
+// Blur/map tile: output kBlurTW x kBlurTH, halo 8 on every side.
+constexpr int kBlurTW = 64;
+constexpr int kBlurTH = 32;
+constexpr int kBlurThreads = 256;
+constexpr int kHalo = 8;
+
+// Partial sums written per blur tile: {sum ssim, sum art, sum det,
+// sum ssim^4, sum art^4, sum det^4}, padded to 8 floats.
+constexpr int kPartialStride = 8;
+
+enum MapFlags : int32_t {
+    kMapSsim = 1,
+    kMapArtifact = 2,
+    kMapDetail = 4,
+};
+
+// Colour front-end description of one image (the device-side restatement of
+// Vship_Colorspace_t, reference src/VshipColor.h:81-96, after validation).
+struct FrontColor {
+    int32_t sample;        // Vship_Sample_t
+    int32_t bits;          // 8..16 for integer samples, 1 for half/float
+    int32_t bytes;         // bytes per sample
+    int32_t range;         // Vship_Range_t
+    int32_t yuv;           // 1 when chroma planes carry the -0.5 / 128 offset
+    int32_t subw, subh;    // log2 chroma decimation
+    int32_t h_cosited;     // horizontal chroma siting: 1 left, 0 centre
+    int32_t v_cosited;     // vertical chroma siting: 1 top, 0 centre
+    int32_t matrix;        // Vship_YUVMatrix_t
+    int32_t transfer;      // Vship_TransferFunction_t
+    int32_t use_prim;      // apply prim[] after linearisation
+    float prim[9];         // row-major 3x3, source primaries -> BT.709
+};
+
+// Pyramid geometry of one pair (both images share it).
+struct PairGeom {
+    int32_t w[kUsedScales];
+    int32_t h[kUsedScales];
+    int32_t pitch[kUsedScales];          // elements
+    int64_t off[kUsedScales][3];         // element offset in the image buffer, -1 if unused
+    int64_t image_elems;                 // elements per image buffer
+};
+
+// One image of one pair for the front-end kernel.
+struct FrontJob {
+    const uint8_t* plane[3];   // device raw planes
+    int64_t stride[3];         // bytes per row
+    void* out;                 // XYB planes of this image (StoreT elements)
+    int32_t width, height;     // level-0 size
+    int32_t tiles_x;           // 64-wide tiles along x
+    int32_t tile_start;        // first global tile index of this job
+    int32_t pair;              // index into the PairGeom table
+    int32_t color;             // index into the FrontColor table
+};
+
+// One active (pair, plane, scale) for the blur/map kernel.
+struct MapJob {
+    const void* a;             // XYB plane of the reference image
+    const void* b;             // XYB plane of the distorted image
+    int32_t w, h, pitch;
+    int32_t flags;             // MapFlags
+    int32_t tiles_x;
+    int32_t tile_start;        // first global tile index == first partial slot
+    int32_t ntiles;
+    int32_t pair;
+    int32_t feature_base;      // plane*36 + scale*6
+    int32_t pad;
+};
+
+// Blur taps (float, like the reference's GaussianHandle, gaussianblur.hpp:7-14)
+// and their prefix integral for the per-axis renormalisation.
+struct BlurTaps {
+    float g[17];
+    float integral[18];
+};
+
+// Per-pair score job for the reduction kernel.
+struct ScoreJob {
+    int32_t job_begin;         // first MapJob of the pair
+    int32_t job_count;
+    int32_t pad[2];
+};
+
+}  // namespace vship
