@@ -115,6 +115,7 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; frame pairs shard across GPUs with one "
@@ -161,6 +162,40 @@ def main():
     traffic_file = os.path.join(ROOT, "profiles", "blur_map_1080p_traffic.json")
     if os.path.exists(traffic_file):
         roofline["traffic"] = json.load(open(traffic_file)).get("bytes_per_launch")
+
+    # batched FP16 storage path on the same 240 pairs: one graph replay per step
+    batched = None
+    if not args.no_batched:
+        cs_list = [cs] * PAIRS
+        p1 = [ptrs[i][0] for i in range(PAIRS)]
+        p2 = [ptrs[i][1] for i in range(PAIRS)]
+        st = [strides] * PAIRS
+        with S.SSIMU2Batch(PAIRS, S.STORAGE_FP16) as bt:
+            tuning, tune_ms = bt.autotune(cs_list, cs_list, p1, p2, st, st, reps=2)
+            log(f"batched fp16 tuning {tuning} ({tune_ms:.3f} ms per batch)")
+            bscores = bt.compute(cs_list, cs_list, p1, p2, st, st)
+            for _ in range(args.warmup):
+                bscores = bt.compute(cs_list, cs_list, p1, p2, st, st)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                bscores = bt.compute(cs_list, cs_list, p1, p2, st, st)
+            b_elapsed = time.perf_counter() - t0
+            b_front, b_blur, b_score, b_graph = bt.profile(10)
+            b_bytes, b_flops = bt.work()
+        b_value = PAIRS * args.steps / b_elapsed
+        blur_bytes16 = 2.0 * 2.0 * 3491280.0 * PAIRS
+        batched = {
+            "value": round(b_value, 2), "unit": "pairs/s", "ratio_vs_plain": round(b_value / value, 3),
+            "storage": "fp16 planes, fp32 arithmetic, fp64 norms and score",
+            "tuning": tuning, "max_abs_delta_vs_plain": round(float(np.max(np.abs(np.array(bscores) - np.array(scores)))), 6),
+            "kernel_ms_per_batch": {"frontend": round(b_front, 4), "blur": round(b_blur, 4), "score": round(b_score, 4),
+                                    "graph": round(b_graph, 4)},
+            "blur_hbm_gbs": round(blur_bytes16 / (b_blur * 1e-3) / 1e9, 1),
+            "blur_hbm_frac": round(blur_bytes16 / (b_blur * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "blur_valu_frac": round(b_flops / (b_blur * 1e-3) / 1e12 / VALU_FP32_PEAK_TFLOPS, 4),
+            "path_gbs": round(b_bytes / (b_graph * 1e-3) / 1e9, 1),
+        }
+        log(f"batched fp16: {b_value:.1f} pairs/s ({b_value / value:.2f}x plain)")
 
     # PCIe-inclusive rate of the reference-shaped call on pinned host planes
     pinned = []
@@ -221,6 +256,7 @@ def main():
                    "handlers": args.handlers},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched_fp16": batched,
         "pcie_inclusive_pairs_per_s": round(host_rate, 2),
         "mean_score": round(float(np.mean(scores)), 5),
     }

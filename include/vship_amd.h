@@ -45,12 +45,81 @@ EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2Profile(Vship_SSIMU2Handler hand
  * computed maps, nblocks = 16x16 blocks over them). */
 EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2Work(Vship_SSIMU2Handler handler, double* bytes, double* flops);
 
+/* ---- batched path: many pairs of uneven sizes per call ---------------- */
+
+typedef struct VshipAMD_SSIMU2Batch {
+    unsigned int id;
+} VshipAMD_SSIMU2Batch;
+
+typedef enum {
+    VshipAMD_StorageFP32 = 0,
+    VshipAMD_StorageFP16 = 1,
+} VshipAMD_Storage;
+
+/* Tuning of the batched kernels: blur tile shape (0: 64x32, 1: 64x16,
+ * 2: 32x64, 3: 32x32, 4: 64x64) and persistent workgroups per CU of the
+ * blur and front-end kernels. */
+typedef struct VshipAMD_BatchTuning {
+    int blur_config;
+    int blur_waves;
+    int front_waves;
+} VshipAMD_BatchTuning;
+
+/* A batch context for up to max_pairs pairs per call on gpu_id. XYB planes
+ * are stored in FP32 or FP16 (arithmetic is FP32, norms and the final score
+ * FP64 on the GPU). Device arenas grow on the first calls and are then
+ * reused; the three kernels and the score read-back are replayed as one HIP
+ * graph. */
+EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2BatchInit(VshipAMD_SSIMU2Batch* batch, int max_pairs, int storage,
+                                                          int gpu_id);
+EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2BatchFree(VshipAMD_SSIMU2Batch batch);
+
+/* Scores of npairs pairs whose planes are resident in the memory of the
+ * batch's GPU. Pair i uses src_cs[i] / dis_cs[i] (any sizes, the two of a
+ * pair equal), planes dsrcp1[3i..3i+2] / dsrcp2[3i..3i+2] and line sizes
+ * lineSize1[3i..] / lineSize2[3i..]. Blocks until scores[0..npairs) are
+ * written. */
+EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2BatchCompute(VshipAMD_SSIMU2Batch batch, int npairs,
+                                                             const Vship_Colorspace_t* src_cs,
+                                                             const Vship_Colorspace_t* dis_cs,
+                                                             const uint8_t* const* dsrcp1, const uint8_t* const* dsrcp2,
+                                                             const int64_t* lineSize1, const int64_t* lineSize2,
+                                                             double* scores);
+
+/* Times every tuning candidate on the given pairs (reps launches each) and
+ * keeps the fastest; returns it and its mean ms per batch. */
+EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2BatchAutotune(VshipAMD_SSIMU2Batch batch, int npairs,
+                                                              const Vship_Colorspace_t* src_cs,
+                                                              const Vship_Colorspace_t* dis_cs,
+                                                              const uint8_t* const* dsrcp1,
+                                                              const uint8_t* const* dsrcp2, const int64_t* lineSize1,
+                                                              const int64_t* lineSize2, int reps,
+                                                              VshipAMD_BatchTuning* chosen, double* best_ms);
+EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2BatchSetTuning(VshipAMD_SSIMU2Batch batch, VshipAMD_BatchTuning tuning);
+EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2BatchGetTuning(VshipAMD_SSIMU2Batch batch, VshipAMD_BatchTuning* tuning);
+
+/* npairs * 108 features of the last batch compute. */
+EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2BatchFeatures(VshipAMD_SSIMU2Batch batch, double* features);
+
+/* Mean device ms per kernel and per whole-graph replay of the last batch. */
+EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2BatchProfile(VshipAMD_SSIMU2Batch batch, int iters,
+                                                             double* ms_frontend, double* ms_blur, double* ms_score,
+                                                             double* ms_graph);
+
+/* Algorithmic HBM bytes and blur-stage FLOPs of the last batch. */
+EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2BatchWork(VshipAMD_SSIMU2Batch batch, double* bytes, double* flops);
+
 /* Device memory helpers for callers without another HIP binding. */
 EXPORTPREPROCESS Vship_Exception VshipAMD_DeviceMalloc(void** ptr, uint64_t size, int gpu_id);
 EXPORTPREPROCESS Vship_Exception VshipAMD_DeviceFree(void* ptr);
 EXPORTPREPROCESS Vship_Exception VshipAMD_MemcpyHtoD(void* dst, const void* src, uint64_t size);
 EXPORTPREPROCESS Vship_Exception VshipAMD_MemcpyDtoH(void* dst, const void* src, uint64_t size);
 EXPORTPREPROCESS Vship_Exception VshipAMD_DeviceSynchronize(void);
+
+/* Copies `bytes` (a multiple of 16) with a plain vector-copy kernel `iters`
+ * times on the null stream and returns the mean ms per copy: an HBM
+ * bandwidth probe and the reference kernel for calibrating byte counters. */
+EXPORTPREPROCESS Vship_Exception VshipAMD_DeviceCopy(void* dst, const void* src, uint64_t bytes, int iters, double* ms);
 
 #ifdef __cplusplus
 }

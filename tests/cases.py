@@ -38,28 +38,64 @@ COLOR_CASES = [
 ]
 
 
-def perturbed_pair(cs, seed: int, pad: int = 0):
-    """Random reference planes and a mildly perturbed distorted copy."""
-    rng = np.random.default_rng(seed)
-    ref, strides = synth.random_planes(cs, rng, pad=pad)
-    dist = []
-    nb = cs.sample_bytes
-    for p, (plane, stride) in enumerate(zip(ref, strides)):
+def _code_values(cs, nominal: np.ndarray, chroma: bool) -> np.ndarray:
+    """Nominal luma [0, 1] / chroma [-0.5, 0.5] values -> stored samples."""
+    bits = {S.SAMPLE_UINT8: 8, 3: 9, 5: 10, 7: 12, 9: 14, 11: 16}.get(cs.sample)
+    if bits is None:  # half / float samples carry full-range values
+        v = nominal + (0.5 if chroma else 0.0)
+        return v.astype(np.float32 if cs.sample == S.SAMPLE_FLOAT else np.float16)
+    if cs.range == S.RANGE_LIMITED:
+        code = (128.0 + 224.0 * nominal if chroma else 16.0 + 219.0 * nominal) * (1 << (bits - 8))
+    else:
+        code = (nominal + (0.5 if chroma else 0.0)) * ((1 << bits) - 1)
+    top = (1 << bits) - 1
+    code = np.clip(np.round(code), 0, top)
+    return code.astype(np.uint8 if bits == 8 else np.uint16)
+
+
+def _pack(cs, planes, pad):
+    out, strides = [], []
+    for p, vals in enumerate(planes):
         h, w = cs.plane_shape(p)
+        raw = np.ascontiguousarray(vals).view(np.uint8).reshape(h, w * cs.sample_bytes)
+        stride = w * cs.sample_bytes + pad * (p + 1)
+        buf = np.zeros((h, stride), dtype=np.uint8)
+        buf[:, :w * cs.sample_bytes] = raw
+        out.append(buf)
+        strides.append(stride)
+    return out, strides
+
+
+def perturbed_pair(cs, seed: int, pad: int = 0, noise: float = 0.025):
+    """In-gamut random reference planes and a perturbed distorted copy.
+
+    Luma (or R, G, B) nominal values ~ U[0.25, 0.75], chroma ~ U[-0.06, 0.06],
+    so every pixel maps to positive linear light; the distorted image adds
+    N(0, noise^2) per sample. Rows are padded by pad*(plane+1) bytes.
+    """
+    rng = np.random.default_rng(seed)
+    yuv = cs.colorFamily == S.FAMILY_YUV
+    ref_nom, dist_nom = [], []
+    for p in range(3):
+        h, w = cs.plane_shape(p)
+        chroma = yuv and p > 0
+        v = rng.uniform(-0.06, 0.06, (h, w)) if chroma else rng.uniform(0.25, 0.75, (h, w))
+        ref_nom.append((v, chroma))
+        dist_nom.append((v + rng.normal(0, noise, (h, w)), chroma))
+    ref, strides = _pack(cs, [_code_values(cs, v, c) for v, c in ref_nom], pad)
+    dist, _ = _pack(cs, [_code_values(cs, v, c) for v, c in dist_nom], pad)
+    return ref, dist, strides
+
+
+def out_of_gamut_pair(cs, seed: int):
+    """Uniform random samples over the whole code range (negative and >1
+    linear light included) and a perturbed copy."""
+    rng = np.random.default_rng(seed)
+    ref, strides = synth.random_planes(cs, rng)
+    dist = []
+    for p, plane in enumerate(ref):
         d = plane.copy()
-        body = d[:, :w * nb]
         if cs.sample == S.SAMPLE_UINT8:
-            v = body.astype(np.int32) + rng.integers(-6, 7, body.shape)
-            d[:, :w * nb] = np.clip(v, 0, 255).astype(np.uint8)
-        elif cs.sample == S.SAMPLE_FLOAT:
-            v = body.view(np.float32) + rng.normal(0, 0.02, (h, w)).astype(np.float32)
-            d[:, :w * nb] = np.clip(v, 0, 1).astype(np.float32).view(np.uint8)
-        elif cs.sample == S.SAMPLE_HALF:
-            v = body.view(np.float16).astype(np.float32) + rng.normal(0, 0.02, (h, w)).astype(np.float32)
-            d[:, :w * nb] = np.clip(v, 0, 1).astype(np.float16).view(np.uint8)
-        else:
-            bits = {3: 9, 5: 10, 7: 12, 9: 14, 11: 16}[cs.sample]
-            v = body.view(np.uint16).astype(np.int64) + rng.integers(-6, 7, (h, w)) * (1 << (bits - 8))
-            d[:, :w * nb] = np.clip(v, 0, (1 << bits) - 1).astype(np.uint16).view(np.uint8)
+            d = np.clip(d.astype(np.int32) + rng.integers(-6, 7, d.shape), 0, 255).astype(np.uint8)
         dist.append(d)
     return ref, dist, strides

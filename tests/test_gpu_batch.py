@@ -1,0 +1,128 @@
+"""GPU tests of the batched path (VshipAMD_SSIMU2Batch*): uneven sizes in one
+call, FP32 storage equal to the plain path, FP16 storage within 0.01 of the
+FP64 oracle, graph replays across changing batch compositions, autotuning."""
+import numpy as np
+import pytest
+
+import oracle
+from vship_amd import ssimu2 as S
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(131, 75), (64, 64), (200, 117), (17, 9), (1, 1), (320, 180), (97, 61), (640, 360)]
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+class DevicePairs:
+    """Seeded pairs uploaded to device memory."""
+
+    def __init__(self, sizes, kinds=None, seed=0):
+        self.cs, self.host, self.strides, self.ptrs, self.bufs = [], [], [], [], []
+        for i, (w, h) in enumerate(sizes):
+            kw = cases.COLOR_CASES[(kinds[i] if kinds else i) % len(cases.COLOR_CASES)][1]
+            cs = S.colorspace(w, h, **kw)
+            ref, dist, st = cases.perturbed_pair(cs, seed=seed + i, pad=3)
+            pair_ptrs = []
+            for img in (ref, dist):
+                trip = []
+                for p in img:
+                    b = S.DeviceBuffer(p.nbytes)
+                    b.upload(p)
+                    self.bufs.append(b)
+                    trip.append(b.address)
+                pair_ptrs.append(trip)
+            self.cs.append(cs)
+            self.host.append((ref, dist))
+            self.strides.append(st)
+            self.ptrs.append(pair_ptrs)
+
+    def args(self, idx=None):
+        idx = range(len(self.cs)) if idx is None else idx
+        return ([self.cs[i] for i in idx], [self.cs[i] for i in idx], [self.ptrs[i][0] for i in idx],
+                [self.ptrs[i][1] for i in idx], [self.strides[i] for i in idx], [self.strides[i] for i in idx])
+
+    def free(self):
+        for b in self.bufs:
+            b.free()
+
+
+@pytest.fixture(scope="module")
+def pairs():
+    kinds = [0, 1, 5, 17, 0, 2, 18, 0]
+    dp = DevicePairs(SIZES, kinds)
+    yield dp
+    dp.free()
+
+
+def test_fp32_batch_equals_plain_path(pairs):
+    with S.SSIMU2Batch(16, S.STORAGE_FP32) as b:
+        got = b.compute(*pairs.args())
+        feats = b.features(len(SIZES))
+    for i, cs in enumerate(pairs.cs):
+        with S.SSIMU2Handler(cs, cs) as h:
+            want = h.compute(pairs.host[i][0], pairs.host[i][1], pairs.strides[i], pairs.strides[i])
+            wf = h.features()
+        _say("fp32 batch", cs.width, cs.height, got[i], want)
+        assert got[i] == want
+        np.testing.assert_array_equal(feats[i], wf)
+
+
+def test_fp16_batch_vs_oracle(pairs):
+    with S.SSIMU2Batch(16, S.STORAGE_FP16) as b:
+        got = b.compute(*pairs.args())
+    worst = 0.0
+    for i, cs in enumerate(pairs.cs):
+        ref, dist = pairs.host[i]
+        want, _ = oracle.ssimu2(cs, ref, pairs.strides[i], cs, dist, pairs.strides[i])
+        _say("fp16 batch", cs.width, cs.height, got[i], want, got[i] - want)
+        worst = max(worst, abs(got[i] - want))
+    assert worst <= 0.01, worst
+
+
+def test_graph_replay_with_changing_compositions(pairs):
+    with S.SSIMU2Batch(16, S.STORAGE_FP16) as b:
+        first = b.compute(*pairs.args())
+        sub = b.compute(*pairs.args([7, 2, 0]))
+        again = b.compute(*pairs.args())
+        one = b.compute(*pairs.args([5]))
+    np.testing.assert_array_equal(first, again)
+    np.testing.assert_array_equal(sub, first[[7, 2, 0]])
+    assert one[0] == first[5]
+
+
+def test_identity_in_batch(pairs):
+    cs, _, p1, _, st, _ = pairs.args()
+    with S.SSIMU2Batch(16, S.STORAGE_FP16) as b:
+        got = b.compute(cs, cs, p1, p1, st, st)
+    assert (got == 100.0).all(), got
+
+
+def test_autotune_keeps_results(pairs):
+    with S.SSIMU2Batch(16, S.STORAGE_FP32) as b:
+        base = b.compute(*pairs.args())
+        tuning, ms = b.autotune(*pairs.args(), reps=1)
+        _say("autotune", tuning, ms)
+        assert 0 <= tuning["blur_config"] < 5 and ms > 0
+        assert b.tuning() == tuning
+        tuned = b.compute(*pairs.args())
+        for cfg in range(5):
+            b.set_tuning(cfg, 2, 2)
+            np.testing.assert_allclose(b.compute(*pairs.args()), base, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(tuned, base, rtol=0, atol=1e-9)
+
+
+def test_batch_errors(pairs):
+    with S.SSIMU2Batch(2, S.STORAGE_FP16) as b:
+        with pytest.raises(S.VshipError):
+            b.compute(*pairs.args([0, 1, 2]))  # more than max_pairs
+        cs = pairs.cs[0]
+        other = S.colorspace(cs.width + 1, cs.height)
+        args = list(pairs.args([0]))
+        args[1] = [other]
+        with pytest.raises(S.VshipError) as e:
+            b.compute(*args)
+        assert e.value.code == 4

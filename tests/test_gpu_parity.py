@@ -39,9 +39,25 @@ def test_colorspaces_vs_oracle(name, kw):
         feats = hd.features()
         ident = hd.compute(ref, ref, strides, strides)
     want, wf = _oracle_score(cs, ref, strides, cs, dist, strides)
+    _say("  gpu", got, "oracle", want, "delta", got - want, "max feature delta", np.abs(feats - wf).max())
     assert abs(got - want) <= SCORE_TOL, (got, want)
     np.testing.assert_allclose(feats, wf, rtol=2e-3, atol=2e-5)
     assert ident == 100.0
+
+
+def test_out_of_gamut_content():
+    # Uniform random 8-bit YUV reaches negative linear light; opsin values then
+    # sit at the max(v, 0) clamp where cbrt has infinite slope, so float
+    # rounding (in the reference as well) moves XYB by up to ~1e-2 there. The
+    # bar is wider and says so.
+    cs = S.colorspace(131, 75)
+    ref, dist, st = cases.out_of_gamut_pair(cs, seed=1)
+    with S.SSIMU2Handler(cs, cs) as hd:
+        got = hd.compute(ref, dist, st, st)
+        assert hd.compute(ref, ref, st, st) == 100.0
+    want, _ = _oracle_score(cs, ref, st, cs, dist, st)
+    _say("out of gamut", got, want)
+    assert abs(got - want) <= 0.05
 
 
 @pytest.mark.parametrize("w,h", [(1, 1), (3, 2), (8, 8), (17, 9), (64, 64), (65, 33), (135, 67), (300, 200)])
@@ -78,6 +94,7 @@ def test_xyb_planes_vs_oracle(name, kw):
                 want = levels[scale][plane]
                 assert got.shape == want.shape
                 err = np.abs(got.astype(np.float64) - want).max()
+                _say(f"  xyb scale {scale} plane {plane}: max abs err {err:.2e}")
                 assert err <= 5e-5, (scale, plane, err)
                 checked += 1
     assert checked == 11
@@ -165,21 +182,25 @@ def test_error_paths_on_gpu():
 
 @pytest.mark.skipif(not refvship.available(), reason="reference libvship not built (oracle/_ref)")
 def test_against_reference_libvship():
+    """GPU vs the reference libvship vs the FP64 oracle on the golden inputs.
+
+    On the synthetic natural-content frames (sRGB 256x256, YUV420 360p..1080p)
+    all three agree within 0.005. On 131x75 uniform-random planes the
+    reference's own float error against the oracle reaches ~0.02 (E[x^2] - mu^2
+    cancels at the smallest scales); there the bar is the GPU against the
+    oracle, and the GPU may differ from the reference by that reference error.
+    """
+    from tests.golden.inputs import golden_inputs
     ref_lib = refvship.RefVship()
-    checks = []
-    ref, dist = synth.srgb_pair(256, 256, seed=0)
-    cs = S.srgb_rgb_full(256, 256)
-    checks.append((cs, ref, [256] * 3, dist))
-    seq = synth.YuvSequence(1920, 1080, frames=2, seed=3)
-    r, d, _ = seq.frame(1)
-    checks.append((S.colorspace(1920, 1080), r, S.host_strides(r), d))
-    for name, kw in cases.COLOR_CASES[:10]:
-        c = S.colorspace(131, 75, **kw)
-        rr, dd, st = cases.perturbed_pair(c, seed=3)
-        checks.append((c, rr, st, dd))
-    for c, rr, st, dd in checks:
-        want = ref_lib.score(c, c, rr, st, dd, st)
-        with S.SSIMU2Handler(c, c) as hd:
-            got = hd.compute(rr, dd, st, st)
-        _say("ref", c.width, c.height, got, want)
-        assert abs(got - want) <= SCORE_TOL, (got, want)
+    for name, cs1, p1, s1, cs2, p2, s2 in golden_inputs():
+        want_ref = ref_lib.score(cs1, cs2, p1, s1, p2, s2)
+        want_or, _ = oracle.ssimu2(cs1, p1, s1, cs2, p2, s2)
+        with S.SSIMU2Handler(cs1, cs2) as hd:
+            got = hd.compute(p1, p2, s1, s2)
+        _say(f"ref {name}: gpu {got:.6f} reference {want_ref:.6f} oracle {want_or:.6f} "
+             f"gpu-oracle {got - want_or:+.2e} ref-oracle {want_ref - want_or:+.2e}")
+        assert abs(got - want_or) <= SCORE_TOL, name
+        if name.startswith("random"):
+            assert abs(got - want_ref) <= abs(want_ref - want_or) + SCORE_TOL, name
+        else:
+            assert abs(got - want_ref) <= SCORE_TOL, name

@@ -18,8 +18,10 @@
 
 #include "../../include/VshipAPI.h"
 #include "../../include/vship_amd.h"
+#include "batch.hpp"
 #include "engine.hpp"
 #include "errors.hpp"
+#include "kernels.hpp"
 
 namespace vship {
 
@@ -64,6 +66,7 @@ struct HandlerSlot {
 
 std::mutex g_lock;
 std::vector<HandlerSlot> g_handlers;
+std::vector<std::unique_ptr<Ssimu2Batch>> g_batches;
 std::string g_last_error = Error(Vship_NoError, __FILE__, __LINE__).message();
 int g_device = 0;
 
@@ -142,6 +145,29 @@ Vship_Exception guarded_handler(unsigned id, F&& f)
         return Vship_BadErrorType;
     }
     return Vship_NoError;
+}
+
+Ssimu2Batch* lookup_batch(unsigned id)
+{
+    std::lock_guard<std::mutex> g(g_lock);
+    if (id >= g_batches.size() || !g_batches[id])
+        throw Error(Vship_BadHandler, __FILE__, __LINE__, "invalid batch id");
+    return g_batches[id].get();
+}
+
+BatchInput make_batch_input(int npairs, const Vship_Colorspace_t* src, const Vship_Colorspace_t* dis,
+                            const uint8_t* const* p1, const uint8_t* const* p2, const int64_t* l1, const int64_t* l2)
+{
+    if (!src || !dis || !p1 || !p2 || !l1 || !l2) VSHIP_THROW(Vship_BadPointer, "null argument");
+    BatchInput in;
+    in.npairs = npairs;
+    in.src = src;
+    in.dis = dis;
+    in.planes1 = p1;
+    in.planes2 = p2;
+    in.ls1 = l1;
+    in.ls2 = l2;
+    return in;
 }
 
 Vship_Exception not_in_library(const char* what)
@@ -447,6 +473,109 @@ Vship_Exception VshipAMD_SSIMU2Work(Vship_SSIMU2Handler handler, double* bytes, 
     });
 }
 
+Vship_Exception VshipAMD_SSIMU2BatchInit(VshipAMD_SSIMU2Batch* batch, int max_pairs, int storage, int gpu_id)
+{
+    if (!batch) return Vship_BadPointer;
+    return guarded([&] {
+        const int n = device_count();
+        if (gpu_id < 0 || gpu_id >= n) VSHIP_THROW(Vship_BadDeviceArgument, "In VshipAMD_SSIMU2BatchInit");
+        if (storage != VshipAMD_StorageFP32 && storage != VshipAMD_StorageFP16)
+            VSHIP_THROW(Vship_BadPointer, "storage must be VshipAMD_StorageFP32 or VshipAMD_StorageFP16");
+        std::unique_ptr<Ssimu2Batch> b(new Ssimu2Batch(max_pairs, storage == VshipAMD_StorageFP16, gpu_id));
+        std::lock_guard<std::mutex> g(g_lock);
+        unsigned id = 0;
+        while (id < g_batches.size() && g_batches[id]) id++;
+        if (id == g_batches.size()) g_batches.emplace_back();
+        g_batches[id] = std::move(b);
+        batch->id = id;
+    });
+}
+
+Vship_Exception VshipAMD_SSIMU2BatchFree(VshipAMD_SSIMU2Batch batch)
+{
+    std::unique_ptr<Ssimu2Batch> b;
+    {
+        std::lock_guard<std::mutex> g(g_lock);
+        if (batch.id >= g_batches.size() || !g_batches[batch.id]) {
+            g_last_error = Error(Vship_BadHandler, __FILE__, __LINE__, "invalid batch id in VshipAMD_SSIMU2BatchFree").message();
+            return Vship_BadHandler;
+        }
+        b = std::move(g_batches[batch.id]);
+    }
+    return guarded([&] { b.reset(); });
+}
+
+Vship_Exception VshipAMD_SSIMU2BatchCompute(VshipAMD_SSIMU2Batch batch, int npairs, const Vship_Colorspace_t* src_cs,
+                                            const Vship_Colorspace_t* dis_cs, const uint8_t* const* dsrcp1,
+                                            const uint8_t* const* dsrcp2, const int64_t* lineSize1,
+                                            const int64_t* lineSize2, double* scores)
+{
+    return guarded([&] {
+        Ssimu2Batch* b = lookup_batch(batch.id);
+        if (!scores) VSHIP_THROW(Vship_BadPointer, "null scores");
+        b->compute(make_batch_input(npairs, src_cs, dis_cs, dsrcp1, dsrcp2, lineSize1, lineSize2), scores);
+    });
+}
+
+Vship_Exception VshipAMD_SSIMU2BatchAutotune(VshipAMD_SSIMU2Batch batch, int npairs, const Vship_Colorspace_t* src_cs,
+                                             const Vship_Colorspace_t* dis_cs, const uint8_t* const* dsrcp1,
+                                             const uint8_t* const* dsrcp2, const int64_t* lineSize1,
+                                             const int64_t* lineSize2, int reps, VshipAMD_BatchTuning* chosen,
+                                             double* best_ms)
+{
+    return guarded([&] {
+        Ssimu2Batch* b = lookup_batch(batch.id);
+        BatchTuning t;
+        b->autotune(make_batch_input(npairs, src_cs, dis_cs, dsrcp1, dsrcp2, lineSize1, lineSize2), reps, &t, best_ms);
+        if (chosen) {
+            chosen->blur_config = t.blur_config;
+            chosen->blur_waves = t.blur_waves;
+            chosen->front_waves = t.front_waves;
+        }
+    });
+}
+
+Vship_Exception VshipAMD_SSIMU2BatchSetTuning(VshipAMD_SSIMU2Batch batch, VshipAMD_BatchTuning tuning)
+{
+    return guarded([&] {
+        BatchTuning t;
+        t.blur_config = tuning.blur_config;
+        t.blur_waves = tuning.blur_waves;
+        t.front_waves = tuning.front_waves;
+        lookup_batch(batch.id)->set_tuning(t);
+    });
+}
+
+Vship_Exception VshipAMD_SSIMU2BatchGetTuning(VshipAMD_SSIMU2Batch batch, VshipAMD_BatchTuning* tuning)
+{
+    if (!tuning) return Vship_BadPointer;
+    return guarded([&] {
+        const BatchTuning t = lookup_batch(batch.id)->tuning();
+        tuning->blur_config = t.blur_config;
+        tuning->blur_waves = t.blur_waves;
+        tuning->front_waves = t.front_waves;
+    });
+}
+
+Vship_Exception VshipAMD_SSIMU2BatchFeatures(VshipAMD_SSIMU2Batch batch, double* features)
+{
+    if (!features) return Vship_BadPointer;
+    return guarded([&] { lookup_batch(batch.id)->features(features); });
+}
+
+Vship_Exception VshipAMD_SSIMU2BatchProfile(VshipAMD_SSIMU2Batch batch, int iters, double* ms_frontend,
+                                            double* ms_blur, double* ms_score, double* ms_graph)
+{
+    if (!ms_frontend || !ms_blur || !ms_score || !ms_graph) return Vship_BadPointer;
+    return guarded([&] { lookup_batch(batch.id)->profile(iters, ms_frontend, ms_blur, ms_score, ms_graph); });
+}
+
+Vship_Exception VshipAMD_SSIMU2BatchWork(VshipAMD_SSIMU2Batch batch, double* bytes, double* flops)
+{
+    if (!bytes || !flops) return Vship_BadPointer;
+    return guarded([&] { lookup_batch(batch.id)->work(bytes, flops); });
+}
+
 Vship_Exception VshipAMD_DeviceMalloc(void** ptr, uint64_t size, int gpu_id)
 {
     if (!ptr) return Vship_BadPointer;
@@ -469,6 +598,25 @@ Vship_Exception VshipAMD_MemcpyHtoD(void* dst, const void* src, uint64_t size)
 Vship_Exception VshipAMD_MemcpyDtoH(void* dst, const void* src, uint64_t size)
 {
     return guarded([&] { VSHIP_HIP_CHECK(hipMemcpy(dst, src, size, hipMemcpyDeviceToHost)); });
+}
+
+Vship_Exception VshipAMD_DeviceCopy(void* dst, const void* src, uint64_t bytes, int iters, double* ms)
+{
+    return guarded([&] {
+        if (!dst || !src || (bytes % 16) != 0 || iters <= 0) VSHIP_THROW(Vship_BadPointer, "bad copy arguments");
+        hipEvent_t e0, e1;
+        VSHIP_HIP_CHECK(hipEventCreate(&e0));
+        VSHIP_HIP_CHECK(hipEventCreate(&e1));
+        VSHIP_HIP_CHECK(hipEventRecord(e0, 0));
+        for (int i = 0; i < iters; i++) VSHIP_HIP_CHECK(launch_copy(dst, src, (int64_t)bytes, 0));
+        VSHIP_HIP_CHECK(hipEventRecord(e1, 0));
+        VSHIP_HIP_CHECK(hipEventSynchronize(e1));
+        float t = 0.0f;
+        VSHIP_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (ms) *ms = t / iters;
+    });
 }
 
 Vship_Exception VshipAMD_DeviceSynchronize(void)

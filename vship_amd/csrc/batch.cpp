@@ -1,0 +1,362 @@
+// batch.cpp - batched, graph-replayed SSIMULACRA2 over pairs of uneven sizes.
+#include "batch.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "errors.hpp"
+#include "kernels.hpp"
+
+namespace vship {
+
+namespace {
+
+constexpr int kScoreGrid = 256;
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+}  // namespace
+
+Ssimu2Batch::Ssimu2Batch(int max_pairs, bool fp16, int device)
+    : device_(device), fp16_(fp16), max_pairs_(max_pairs)
+{
+    if (max_pairs <= 0 || max_pairs > (1 << 20)) VSHIP_THROW(Vship_BadPointer, "max_pairs must be in [1, 2^20]");
+    taps_ = make_blur_taps();
+    VSHIP_HIP_CHECK(hipSetDevice(device_));
+    hipDeviceProp_t prop;
+    VSHIP_HIP_CHECK(hipGetDeviceProperties(&prop, device_));
+    cus_ = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    VSHIP_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+
+    // fixed-capacity packed tables: counts | front jobs | colours | geoms | maps | score jobs
+    cap_fjobs_ = 2 * max_pairs;
+    cap_maps_ = kMaxJobsPerPair * max_pairs;
+    size_t off = 0;
+    const size_t o_counts = off; off = align_up(off + kCountSlots * sizeof(int), 256);
+    const size_t o_fjobs = off; off = align_up(off + cap_fjobs_ * sizeof(FrontJob), 256);
+    const size_t o_colors = off; off = align_up(off + cap_fjobs_ * sizeof(FrontColor), 256);
+    const size_t o_geoms = off; off = align_up(off + max_pairs * sizeof(PairGeom), 256);
+    const size_t o_maps = off; off = align_up(off + cap_maps_ * sizeof(MapJob), 256);
+    const size_t o_sjobs = off; off = align_up(off + max_pairs * sizeof(ScoreJob), 256);
+    tables_cap_ = off;
+    if (hipMalloc((void**)&tables_, tables_cap_) != hipSuccess ||
+        hipMalloc((void**)&d_features_, (size_t)max_pairs * 108 * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&d_scores_, (size_t)max_pairs * sizeof(double)) != hipSuccess)
+        VSHIP_THROW(Vship_OutOfVRAM, "batch table allocation failed");
+    if (hipHostMalloc((void**)&h_tables_, tables_cap_, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&h_scores_, (size_t)max_pairs * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        VSHIP_THROW(Vship_OutOfRAM, "batch pinned allocation failed");
+    d_counts_ = reinterpret_cast<int*>(tables_ + o_counts);
+    d_fjobs_ = reinterpret_cast<FrontJob*>(tables_ + o_fjobs);
+    d_colors_ = reinterpret_cast<FrontColor*>(tables_ + o_colors);
+    d_geoms_ = reinterpret_cast<PairGeom*>(tables_ + o_geoms);
+    d_maps_ = reinterpret_cast<MapJob*>(tables_ + o_maps);
+    d_sjobs_ = reinterpret_cast<ScoreJob*>(tables_ + o_sjobs);
+}
+
+Ssimu2Batch::~Ssimu2Batch()
+{
+    (void)hipSetDevice(device_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    (void)hipFree(xyb_);
+    (void)hipFree(partials_);
+    (void)hipFree(tables_);
+    (void)hipFree(d_features_);
+    (void)hipFree(d_scores_);
+    (void)hipHostFree(h_tables_);
+    (void)hipHostFree(h_scores_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Ssimu2Batch::build_tables(const BatchInput& in)
+{
+    if (in.npairs <= 0 || in.npairs > max_pairs_) VSHIP_THROW(Vship_BadPointer, "npairs outside [1, max_pairs]");
+    fjobs_.clear();
+    colors_.clear();
+    geoms_.clear();
+    maps_.clear();
+    sjobs_.clear();
+    xyb_off_.clear();
+    const size_t elem = fp16_ ? 2 : 4;
+    int64_t elems = 0;
+    int ftiles = 0, btiles = 0;
+    double in_bytes = 0.0, active = 0.0, flops = 0.0, blocks = 0.0;
+    std::vector<PairLayout> layouts;
+    layouts.reserve(in.npairs);
+    for (int i = 0; i < in.npairs; i++) {
+        const ImageSpec s1 = make_image_spec(in.src[i]);
+        const ImageSpec s2 = make_image_spec(in.dis[i]);
+        if (s1.width != s2.width || s1.height != s2.height)
+            VSHIP_THROW(Vship_DifferingInputType, "pair " + std::to_string(i) + ": sizes differ");
+        const ImageSpec* specs[2] = {&s1, &s2};
+        const uint8_t* const* pl[2] = {in.planes1 + 3 * i, in.planes2 + 3 * i};
+        const int64_t* ls[2] = {in.ls1 + 3 * i, in.ls2 + 3 * i};
+        for (int k = 0; k < 2; k++)
+            for (int p = 0; p < 3; p++) {
+                if (pl[k][p] == nullptr) VSHIP_THROW(Vship_BadPointer, "null plane pointer");
+                if (ls[k][p] < specs[k]->plane_w[p] * specs[k]->fc.bytes)
+                    VSHIP_THROW(Vship_BadPointer, "lineSize smaller than one row of samples");
+                in_bytes += (double)specs[k]->plane_w[p] * specs[k]->plane_h[p] * specs[k]->fc.bytes;
+            }
+        layouts.push_back(make_pair_layout((int)s1.width, (int)s1.height, tuning_.blur_config));
+        const PairLayout& L = layouts.back();
+        geoms_.push_back(L.geom);
+        for (int k = 0; k < 2; k++) {
+            // keep every image buffer 256-byte aligned
+            elems = (int64_t)align_up((size_t)elems, 256 / elem);
+            xyb_off_.push_back(elems);
+            elems += L.geom.image_elems;
+            FrontJob j;
+            std::memset(&j, 0, sizeof(j));
+            for (int p = 0; p < 3; p++) {
+                j.plane[p] = pl[k][p];
+                j.stride[p] = ls[k][p];
+            }
+            j.width = (int32_t)s1.width;
+            j.height = (int32_t)s1.height;
+            j.tiles_x = L.front_tiles_x;
+            j.tile_start = ftiles;
+            j.pair = i;
+            j.color = (int32_t)colors_.size();
+            colors_.push_back(specs[k]->fc);
+            ftiles += L.front_tiles_per_image;
+            fjobs_.push_back(j);
+        }
+        ScoreJob sj = {(int32_t)maps_.size(), (int32_t)L.maps.size(), {0, 0}};
+        sjobs_.push_back(sj);
+        for (const MapJob& m0 : L.maps) {
+            MapJob m = m0;
+            m.pair = i;
+            m.tile_start = btiles + m0.tile_start;
+            // a/b hold element offsets until the arena is known
+            m.a = reinterpret_cast<const void*>(xyb_off_[2 * i] + reinterpret_cast<int64_t>(m0.a));
+            m.b = reinterpret_cast<const void*>(xyb_off_[2 * i + 1] + reinterpret_cast<int64_t>(m0.b));
+            maps_.push_back(m);
+            const double px = (double)m.w * m.h;
+            active += px;
+            const bool ssim = m.flags & kMapSsim;
+            const bool edge = m.flags & (kMapArtifact | kMapDetail);
+            const int n_edge = ((m.flags & kMapArtifact) ? 1 : 0) + ((m.flags & kMapDetail) ? 1 : 0);
+            flops += px * (70.0 * (ssim ? 4 : 2) + 23.0 * ssim + 10.0 * edge + 4.0 * n_edge);
+            blocks += std::ceil(m.w / 16.0) * std::ceil(m.h / 16.0);
+        }
+        btiles += L.blur_tiles;
+    }
+    xyb_elems_ = elems;
+    front_tiles_ = ftiles;
+    blur_tiles_ = btiles;
+    npairs_ = in.npairs;
+    work_bytes_ = in_bytes + 2.0 * 2.0 * (double)elem * active + 48.0 * blocks;
+    work_flops_ = flops;
+}
+
+void Ssimu2Batch::ensure_capacity()
+{
+    const size_t elem = fp16_ ? 2 : 4;
+    if (xyb_elems_ > xyb_cap_) {
+        (void)hipFree(xyb_);
+        xyb_ = nullptr;
+        xyb_cap_ = 0;
+        const int64_t want = xyb_elems_ + xyb_elems_ / 8;  // headroom for the next batches
+        if (hipMalloc(&xyb_, (size_t)want * elem) != hipSuccess)
+            VSHIP_THROW(Vship_OutOfVRAM, "batch XYB arena allocation failed");
+        xyb_cap_ = want;
+    }
+    if (blur_tiles_ > partials_cap_) {
+        (void)hipFree(partials_);
+        partials_ = nullptr;
+        partials_cap_ = 0;
+        const int64_t want = blur_tiles_ + blur_tiles_ / 8 + 64;
+        if (hipMalloc((void**)&partials_, (size_t)want * kPartialStride * sizeof(float)) != hipSuccess)
+            VSHIP_THROW(Vship_OutOfVRAM, "batch partial-sum allocation failed");
+        partials_cap_ = want;
+        graph_valid_ = false;  // the kernels take the partials pointer as an argument
+    }
+    // final pointers into the arena, then one packed upload
+    for (auto& j : fjobs_) {
+        const int64_t off = xyb_off_[&j - fjobs_.data()];
+        j.out = static_cast<uint8_t*>(xyb_) + off * elem;
+    }
+    std::vector<MapJob> dev_maps = maps_;
+    for (auto& m : dev_maps) {
+        m.a = static_cast<const uint8_t*>(xyb_) + reinterpret_cast<int64_t>(m.a) * elem;
+        m.b = static_cast<const uint8_t*>(xyb_) + reinterpret_cast<int64_t>(m.b) * elem;
+    }
+    int counts[kCountSlots] = {};
+    counts[kCountFrontTiles] = front_tiles_;
+    counts[kCountFrontJobs] = (int)fjobs_.size();
+    counts[kCountBlurTiles] = blur_tiles_;
+    counts[kCountBlurJobs] = (int)dev_maps.size();
+    counts[kCountPairs] = npairs_;
+    auto put = [&](const void* dev, const void* src, size_t bytes) {
+        std::memcpy(h_tables_ + (static_cast<const uint8_t*>(dev) - tables_), src, bytes);
+    };
+    put(d_counts_, counts, sizeof(counts));
+    put(d_fjobs_, fjobs_.data(), fjobs_.size() * sizeof(FrontJob));
+    put(d_colors_, colors_.data(), colors_.size() * sizeof(FrontColor));
+    put(d_geoms_, geoms_.data(), geoms_.size() * sizeof(PairGeom));
+    put(d_maps_, dev_maps.data(), dev_maps.size() * sizeof(MapJob));
+    put(d_sjobs_, sjobs_.data(), sjobs_.size() * sizeof(ScoreJob));
+    const size_t used = (reinterpret_cast<uint8_t*>(d_sjobs_) - tables_) + sjobs_.size() * sizeof(ScoreJob);
+    VSHIP_HIP_CHECK(hipMemcpyAsync(tables_, h_tables_, used, hipMemcpyHostToDevice, stream_));
+}
+
+void Ssimu2Batch::launch_kernels(hipStream_t s)
+{
+    const int fgrid = std::max(1, tuning_.front_waves) * cus_;
+    const int bgrid = (std::max(1, tuning_.blur_waves) * cus_ + 7) / 8 * 8;
+    const size_t lds = frontend_max_lds_bytes();
+    if (fp16_) {
+        VSHIP_HIP_CHECK(launch_frontend<__half>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, s));
+        VSHIP_HIP_CHECK(launch_blur<__half>(tuning_.blur_config, d_maps_, d_counts_, bgrid, taps_, partials_, s));
+    } else {
+        VSHIP_HIP_CHECK(launch_frontend<float>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, s));
+        VSHIP_HIP_CHECK(launch_blur<float>(tuning_.blur_config, d_maps_, d_counts_, bgrid, taps_, partials_, s));
+    }
+    VSHIP_HIP_CHECK(launch_score(d_sjobs_, d_counts_, kScoreGrid, d_maps_, partials_, d_features_, d_scores_, s));
+}
+
+void Ssimu2Batch::capture()
+{
+    if (exec_) { (void)hipGraphExecDestroy(exec_); exec_ = nullptr; }
+    if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+    VSHIP_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    try {
+        launch_kernels(stream_);
+        VSHIP_HIP_CHECK(hipMemcpyAsync(h_scores_, d_scores_, (size_t)max_pairs_ * sizeof(double),
+                                       hipMemcpyDeviceToHost, stream_));
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(stream_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    VSHIP_HIP_CHECK(hipStreamEndCapture(stream_, &graph_));
+    VSHIP_HIP_CHECK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+    graph_valid_ = true;
+}
+
+void Ssimu2Batch::compute(const BatchInput& in, double* scores)
+{
+    VSHIP_HIP_CHECK(hipSetDevice(device_));
+    build_tables(in);
+    ensure_capacity();
+    if (!graph_valid_) capture();
+    VSHIP_HIP_CHECK(hipGraphLaunch(exec_, stream_));
+    VSHIP_HIP_CHECK(hipStreamSynchronize(stream_));
+    std::memcpy(scores, h_scores_, (size_t)npairs_ * sizeof(double));
+}
+
+void Ssimu2Batch::features(double* out)
+{
+    VSHIP_HIP_CHECK(hipSetDevice(device_));
+    VSHIP_HIP_CHECK(hipMemcpy(out, d_features_, (size_t)npairs_ * 108 * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+void Ssimu2Batch::set_tuning(const BatchTuning& t)
+{
+    if (t.blur_config < 0 || t.blur_config >= blur_tile_configs() || t.blur_waves < 1 || t.blur_waves > 16 ||
+        t.front_waves < 1 || t.front_waves > 16)
+        VSHIP_THROW(Vship_BadPointer, "tuning parameters out of range");
+    tuning_ = t;
+    graph_valid_ = false;
+}
+
+void Ssimu2Batch::autotune(const BatchInput& in, int reps, BatchTuning* chosen, double* best_ms)
+{
+    VSHIP_HIP_CHECK(hipSetDevice(device_));
+    reps = std::max(1, reps);
+    hipEvent_t e0, e1;
+    VSHIP_HIP_CHECK(hipEventCreate(&e0));
+    VSHIP_HIP_CHECK(hipEventCreate(&e1));
+    auto time_config = [&](const BatchTuning& t) {
+        tuning_ = t;
+        build_tables(in);
+        ensure_capacity();
+        launch_kernels(stream_);  // warm-up
+        VSHIP_HIP_CHECK(hipEventRecord(e0, stream_));
+        for (int r = 0; r < reps; r++) launch_kernels(stream_);
+        VSHIP_HIP_CHECK(hipEventRecord(e1, stream_));
+        VSHIP_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        VSHIP_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        return (double)ms / reps;
+    };
+    BatchTuning best = tuning_;
+    double best_t = 1e30;
+    // blur tile shape and persistent blur grid, front-end grid fixed
+    for (int cfg = 0; cfg < blur_tile_configs(); cfg++)
+        for (int bw : {1, 2, 3, 4}) {
+            BatchTuning t = best_t < 1e30 ? best : tuning_;
+            t.blur_config = cfg;
+            t.blur_waves = bw;
+            const double ms = time_config(t);
+            if (ms < best_t) { best_t = ms; best = t; }
+        }
+    for (int fw : {1, 2, 4, 6, 8}) {
+        BatchTuning t = best;
+        t.front_waves = fw;
+        const double ms = time_config(t);
+        if (ms < best_t) { best_t = ms; best = t; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    tuning_ = best;
+    graph_valid_ = false;
+    if (chosen) *chosen = best;
+    if (best_ms) *best_ms = best_t;
+}
+
+void Ssimu2Batch::profile(int iters, double* ms_front, double* ms_blur, double* ms_score, double* ms_graph)
+{
+    if (npairs_ == 0) VSHIP_THROW(Vship_BadPointer, "profile needs one compute call first");
+    VSHIP_HIP_CHECK(hipSetDevice(device_));
+    iters = std::max(1, iters);
+    hipEvent_t ev[4];
+    for (auto& e : ev) VSHIP_HIP_CHECK(hipEventCreate(&e));
+    const int fgrid = std::max(1, tuning_.front_waves) * cus_;
+    const int bgrid = (std::max(1, tuning_.blur_waves) * cus_ + 7) / 8 * 8;
+    const size_t lds = frontend_max_lds_bytes();
+    double acc[3] = {0, 0, 0};
+    for (int it = 0; it < iters; it++) {
+        VSHIP_HIP_CHECK(hipEventRecord(ev[0], stream_));
+        if (fp16_) VSHIP_HIP_CHECK(launch_frontend<__half>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, stream_));
+        else VSHIP_HIP_CHECK(launch_frontend<float>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, stream_));
+        VSHIP_HIP_CHECK(hipEventRecord(ev[1], stream_));
+        if (fp16_) VSHIP_HIP_CHECK(launch_blur<__half>(tuning_.blur_config, d_maps_, d_counts_, bgrid, taps_, partials_, stream_));
+        else VSHIP_HIP_CHECK(launch_blur<float>(tuning_.blur_config, d_maps_, d_counts_, bgrid, taps_, partials_, stream_));
+        VSHIP_HIP_CHECK(hipEventRecord(ev[2], stream_));
+        VSHIP_HIP_CHECK(launch_score(d_sjobs_, d_counts_, kScoreGrid, d_maps_, partials_, d_features_, d_scores_, stream_));
+        VSHIP_HIP_CHECK(hipEventRecord(ev[3], stream_));
+        VSHIP_HIP_CHECK(hipEventSynchronize(ev[3]));
+        for (int k = 0; k < 3; k++) {
+            float ms = 0.0f;
+            VSHIP_HIP_CHECK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+            acc[k] += ms;
+        }
+    }
+    // whole graph replays
+    if (!graph_valid_) capture();
+    VSHIP_HIP_CHECK(hipEventRecord(ev[0], stream_));
+    for (int it = 0; it < iters; it++) VSHIP_HIP_CHECK(hipGraphLaunch(exec_, stream_));
+    VSHIP_HIP_CHECK(hipEventRecord(ev[1], stream_));
+    VSHIP_HIP_CHECK(hipEventSynchronize(ev[1]));
+    float gms = 0.0f;
+    VSHIP_HIP_CHECK(hipEventElapsedTime(&gms, ev[0], ev[1]));
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    *ms_front = acc[0] / iters;
+    *ms_blur = acc[1] / iters;
+    *ms_score = acc[2] / iters;
+    *ms_graph = gms / iters;
+}
+
+void Ssimu2Batch::work(double* bytes, double* flops) const
+{
+    *bytes = work_bytes_;
+    *flops = work_flops_;
+}
+
+}  // namespace vship

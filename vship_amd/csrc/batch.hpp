@@ -1,0 +1,111 @@
+// batch.hpp - batched SSIMULACRA2 over many frame pairs of uneven sizes.
+//
+// One Ssimu2Batch owns device arenas (XYB planes of every image, tile
+// partials, features, scores) and a HIP graph of the three persistent kernels
+// plus the score read-back. A compute call builds the work tables of all pairs
+// on the host (front-end tiles, active maps, pair geometry), uploads them in
+// one copy, and replays the graph: the kernels read their item counts from
+// device memory, so one captured graph serves any batch composition that fits
+// the arenas. Planes are stored in FP16 (or FP32), all arithmetic is FP32, and
+// the norms, the 108-weight sum and the polynomial are FP64 on the GPU.
+//
+// The autotuner times every blur tile shape and persistent grid size on a
+// caller-supplied batch and keeps the fastest; the graph is re-captured with
+// it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <vector>
+
+#include "../../include/VshipAPI.h"
+#include "engine.hpp"
+#include "ssimu2_common.hpp"
+
+namespace vship {
+
+struct BatchTuning {
+    int blur_config = 0;     // blur tile shape (blur_tile_shape)
+    int blur_waves = 2;      // persistent blur workgroups per CU
+    int front_waves = 4;     // persistent front-end workgroups per CU
+};
+
+struct BatchInput {
+    int npairs;
+    const Vship_Colorspace_t* src;    // npairs descriptors
+    const Vship_Colorspace_t* dis;
+    const uint8_t* const* planes1;    // npairs * 3 device pointers
+    const uint8_t* const* planes2;
+    const int64_t* ls1;               // npairs * 3 line sizes (bytes)
+    const int64_t* ls2;
+};
+
+class Ssimu2Batch {
+public:
+    Ssimu2Batch(int max_pairs, bool fp16, int device);
+    ~Ssimu2Batch();
+    Ssimu2Batch(const Ssimu2Batch&) = delete;
+    Ssimu2Batch& operator=(const Ssimu2Batch&) = delete;
+
+    int device() const { return device_; }
+    void compute(const BatchInput& in, double* scores);
+    void features(double* out);            // npairs * 108 of the last compute
+    void autotune(const BatchInput& in, int reps, BatchTuning* chosen, double* best_ms);
+    void set_tuning(const BatchTuning& t);
+    BatchTuning tuning() const { return tuning_; }
+    // mean device ms of each kernel and of the whole graph over `iters` replays
+    void profile(int iters, double* ms_front, double* ms_blur, double* ms_score, double* ms_graph);
+    void work(double* bytes, double* flops) const;
+
+private:
+    void build_tables(const BatchInput& in);
+    void ensure_capacity();
+    void capture();
+    void launch_kernels(hipStream_t s);
+
+    int device_;
+    bool fp16_;
+    int max_pairs_;
+    int cus_ = 256;
+    hipStream_t stream_ = nullptr;
+    BatchTuning tuning_;
+    BlurTaps taps_;
+
+    // host tables of the current batch
+    std::vector<FrontJob> fjobs_;
+    std::vector<FrontColor> colors_;
+    std::vector<PairGeom> geoms_;
+    std::vector<MapJob> maps_;
+    std::vector<ScoreJob> sjobs_;
+    std::vector<int64_t> xyb_off_;   // per image (2 per pair): element offset in the arena
+    int64_t xyb_elems_ = 0;
+    int front_tiles_ = 0, blur_tiles_ = 0, npairs_ = 0;
+    double work_bytes_ = 0.0, work_flops_ = 0.0;
+
+    // device
+    void* xyb_ = nullptr;
+    int64_t xyb_cap_ = 0;            // elements
+    float* partials_ = nullptr;
+    int64_t partials_cap_ = 0;       // tiles
+    uint8_t* tables_ = nullptr;      // packed tables
+    size_t tables_cap_ = 0;
+    uint8_t* h_tables_ = nullptr;    // pinned staging of the packed tables
+    double* d_features_ = nullptr;
+    double* d_scores_ = nullptr;
+    double* h_scores_ = nullptr;     // pinned
+    // table views inside tables_
+    int* d_counts_ = nullptr;
+    FrontJob* d_fjobs_ = nullptr;
+    FrontColor* d_colors_ = nullptr;
+    PairGeom* d_geoms_ = nullptr;
+    MapJob* d_maps_ = nullptr;
+    ScoreJob* d_sjobs_ = nullptr;
+    size_t tables_bytes_ = 0;
+    int cap_fjobs_ = 0, cap_maps_ = 0;
+
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+    bool graph_valid_ = false;
+};
+
+}  // namespace vship

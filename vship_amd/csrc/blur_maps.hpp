@@ -14,6 +14,9 @@ struct MapSums {
 
 // SSIM map value, reference src/ssimu2/score.hpp:266-275, with
 // s11 + s22 = G(a^2 + b^2) blurred directly instead of G((a+b)^2) - 2 G(ab).
+// All inputs may be centred on one constant (m = mu - c, s12 = G((a-c)(b-c)),
+// sq = G((a-c)^2 + (b-c)^2)): the formula only uses mu1 - mu2 and the
+// (co)variances, which centring leaves unchanged.
 // For a == b: sq == 2 * s12 and m11 + m22 == 2 * m12 exactly, so num_s and
 // den_s are the same float and the map is exactly 0.
 __device__ inline float ssim_value(float m1, float m2, float s12, float sq)
@@ -30,7 +33,8 @@ __device__ inline float ssim_value(float m1, float m2, float s12, float sq)
 }
 
 // Edge difference d = (|b - mu2| + 1) / (|a - mu1| + 1) - 1, artifact =
-// max(d, 0), detail loss = max(-d, 0) (reference score.hpp:276-281).
+// max(d, 0), detail loss = max(-d, 0) (reference score.hpp:276-281); a, b,
+// mu1, mu2 may all be centred on the same constant.
 __device__ inline float edge_value(float a, float b, float m1, float m2)
 {
 #pragma clang fp contract(off)

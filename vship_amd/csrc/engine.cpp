@@ -164,8 +164,9 @@ BlurTaps make_blur_taps()
     return t;
 }
 
-PairLayout make_pair_layout(int w, int h)
+PairLayout make_pair_layout(int w, int h, int blur_config)
 {
+    const BlurTile bt = blur_tile_shape(blur_config);
     PairLayout L;
     std::memset(&L.geom, 0, sizeof(L.geom));
     int64_t cw = w, ch = h, off = 0;
@@ -201,8 +202,8 @@ PairLayout make_pair_layout(int w, int h)
             m.pitch = L.geom.pitch[s];
             m.flags = (map_active(p, s, 0) ? kMapSsim : 0) | (map_active(p, s, 1) ? kMapArtifact : 0) |
                       (map_active(p, s, 2) ? kMapDetail : 0);
-            m.tiles_x = (m.w + kBlurTW - 1) / kBlurTW;
-            m.ntiles = m.tiles_x * ((m.h + kBlurTH - 1) / kBlurTH);
+            m.tiles_x = (m.w + bt.tw - 1) / bt.tw;
+            m.ntiles = m.tiles_x * ((m.h + bt.th - 1) / bt.th);
             m.tile_start = tiles;
             m.pair = 0;
             m.feature_base = p * 36 + s * 6;
@@ -211,6 +212,7 @@ PairLayout make_pair_layout(int w, int h)
         }
     }
     L.blur_tiles = tiles;
+    L.blur_config = blur_config;
     return L;
 }
 
@@ -239,7 +241,7 @@ Ssimu2Handler::Ssimu2Handler(const Vship_Colorspace_t& src, const Vship_Colorspa
     dmalloc((void**)&d_geom_, sizeof(PairGeom));
     dmalloc((void**)&d_maps_, layout_.maps.size() * sizeof(MapJob));
     dmalloc((void**)&d_score_jobs_, sizeof(ScoreJob));
-    dmalloc((void**)&d_counts_, 4 * sizeof(int));
+    dmalloc((void**)&d_counts_, kCountSlots * sizeof(int));
     dmalloc((void**)&d_partials_, (size_t)layout_.blur_tiles * kPartialStride * sizeof(float));
     dmalloc((void**)&d_features_, 108 * sizeof(double));
     dmalloc((void**)&d_score_, sizeof(double));
@@ -256,7 +258,12 @@ Ssimu2Handler::Ssimu2Handler(const Vship_Colorspace_t& src, const Vship_Colorspa
     }
     FrontColor colors[2] = {spec_[0].fc, spec_[1].fc};
     ScoreJob sj = {0, (int32_t)maps.size(), {0, 0}};
-    const int counts[4] = {2 * layout_.front_tiles_per_image, layout_.blur_tiles, 1, 0};
+    int counts[kCountSlots] = {};
+    counts[kCountFrontTiles] = 2 * layout_.front_tiles_per_image;
+    counts[kCountFrontJobs] = 2;
+    counts[kCountBlurTiles] = layout_.blur_tiles;
+    counts[kCountBlurJobs] = (int)maps.size();
+    counts[kCountPairs] = 1;
     VSHIP_HIP_CHECK(hipMemcpy(d_maps_, maps.data(), maps.size() * sizeof(MapJob), hipMemcpyHostToDevice));
     VSHIP_HIP_CHECK(hipMemcpy(d_colors_, colors, sizeof(colors), hipMemcpyHostToDevice));
     VSHIP_HIP_CHECK(hipMemcpy(d_geom_, &layout_.geom, sizeof(PairGeom), hipMemcpyHostToDevice));
@@ -319,11 +326,11 @@ void Ssimu2Handler::ensure_staging(int image, const int64_t ls[3])
 void Ssimu2Handler::launch_all()
 {
     VSHIP_HIP_CHECK(hipMemcpyAsync(d_front_jobs_, h_front_jobs_, 2 * sizeof(FrontJob), hipMemcpyHostToDevice, stream_));
-    VSHIP_HIP_CHECK(launch_frontend<float>(d_front_jobs_, 2, d_counts_ + 0, 2 * layout_.front_tiles_per_image,
-                                           d_colors_, d_geom_, front_lds_, stream_));
-    VSHIP_HIP_CHECK(launch_blur<float>(d_maps_, (int)layout_.maps.size(), d_counts_ + 1, layout_.blur_tiles, taps_,
-                                       d_partials_, stream_));
-    VSHIP_HIP_CHECK(launch_score(d_score_jobs_, d_counts_ + 2, 1, d_maps_, d_partials_, d_features_, d_score_, stream_));
+    VSHIP_HIP_CHECK(launch_frontend<float>(d_front_jobs_, d_counts_, 2 * layout_.front_tiles_per_image, d_colors_,
+                                           d_geom_, front_lds_, stream_));
+    VSHIP_HIP_CHECK(launch_blur<float>(layout_.blur_config, d_maps_, d_counts_, blur_grid_for(layout_.blur_tiles),
+                                       taps_, d_partials_, stream_));
+    VSHIP_HIP_CHECK(launch_score(d_score_jobs_, d_counts_, 1, d_maps_, d_partials_, d_features_, d_score_, stream_));
 }
 
 void Ssimu2Handler::run(const uint8_t* const p1[3], const uint8_t* const p2[3], const int64_t ls1[3],
@@ -405,13 +412,13 @@ void Ssimu2Handler::profile(int iters, double* ms_front, double* ms_blur, double
     double acc[3] = {0, 0, 0};
     for (int it = 0; it < iters; it++) {
         VSHIP_HIP_CHECK(hipEventRecord(ev[0], stream_));
-        VSHIP_HIP_CHECK(launch_frontend<float>(d_front_jobs_, 2, d_counts_ + 0, 2 * layout_.front_tiles_per_image,
+        VSHIP_HIP_CHECK(launch_frontend<float>(d_front_jobs_, d_counts_, 2 * layout_.front_tiles_per_image,
                                                d_colors_, d_geom_, front_lds_, stream_));
         VSHIP_HIP_CHECK(hipEventRecord(ev[1], stream_));
-        VSHIP_HIP_CHECK(launch_blur<float>(d_maps_, (int)layout_.maps.size(), d_counts_ + 1, layout_.blur_tiles,
+        VSHIP_HIP_CHECK(launch_blur<float>(layout_.blur_config, d_maps_, d_counts_, blur_grid_for(layout_.blur_tiles),
                                            taps_, d_partials_, stream_));
         VSHIP_HIP_CHECK(hipEventRecord(ev[2], stream_));
-        VSHIP_HIP_CHECK(launch_score(d_score_jobs_, d_counts_ + 2, 1, d_maps_, d_partials_, d_features_, d_score_,
+        VSHIP_HIP_CHECK(launch_score(d_score_jobs_, d_counts_, 1, d_maps_, d_partials_, d_features_, d_score_,
                                      stream_));
         VSHIP_HIP_CHECK(hipEventRecord(ev[3], stream_));
         VSHIP_HIP_CHECK(hipEventSynchronize(ev[3]));

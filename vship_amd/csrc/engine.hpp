@@ -39,9 +39,10 @@ struct PairLayout {
     int front_tiles_x;
     std::vector<MapJob> maps;     // a/b filled with element offsets, not pointers
     int blur_tiles;
+    int blur_config;
 };
 
-PairLayout make_pair_layout(int w, int h);
+PairLayout make_pair_layout(int w, int h, int blur_config = 0);
 
 class Ssimu2Handler {
 public:

@@ -58,33 +58,31 @@ __device__ inline void store_row4(StoreT* dst, const float v[4], int xg, int wid
         if (xg + i < width) dst[i] = store_cast<StoreT>(v[i]);
 }
 
-template <typename StoreT>
-__global__ __launch_bounds__(kFrontThreads)
-void frontend_kernel(const FrontJob* __restrict__ jobs, int njobs, const int* __restrict__ total_tiles,
-                     const FrontColor* __restrict__ colors, const PairGeom* __restrict__ geoms)
+template <int KIND>
+__device__ inline float load_kind(const uint8_t* p, uint32_t mask)
 {
-    extern __shared__ float lds[];
+    if constexpr (KIND == kSampleU8) {
+        return (float)p[0];
+    } else if constexpr (KIND == kSampleU16) {
+        return (float)(((uint32_t)p[0] | ((uint32_t)p[1] << 8)) & mask);
+    } else if constexpr (KIND == kSampleHalf) {
+        return __half2float(__ushort_as_half((uint16_t)((uint32_t)p[0] | ((uint32_t)p[1] << 8))));
+    } else {
+        return __uint_as_float((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24));
+    }
+}
+
+template <typename StoreT, int KIND>
+__device__ __attribute__((always_inline)) void front_tile(const FrontJob& job, const FrontColor& fc, const PairGeom& geo, int tile_x, int tile_y,
+                           float* lds)
+{
+    constexpr int kBytes = (KIND == kSampleU8) ? 1 : (KIND == kSampleFloat ? 4 : 2);
     const int t = threadIdx.x;
     const int tx = t & 15, ty = t >> 4;
-    const int total = *total_tiles;
-
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
-        // job lookup: largest j with tile_start <= item
-        int lo = 0, hi = njobs - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (jobs[mid].tile_start <= item) lo = mid; else hi = mid - 1;
-        }
-        const FrontJob& job = jobs[lo];
-        const FrontColor fc = colors[job.color];
-        const PairGeom& geo = geoms[job.pair];
-        const int local = item - job.tile_start;
-        const int tile_x = local % job.tiles_x;
-        const int tile_y = local / job.tiles_x;
-        const int W = job.width, H = job.height;
-        const int x0 = tile_x * kFrontTile + 4 * tx;
-        const int y0 = tile_y * kFrontTile + 4 * ty;
-        const uint32_t mask = (fc.bits >= 16) ? 0xffffu : ((1u << fc.bits) - 1u);
+    const int W = job.width, H = job.height;
+    const int x0 = tile_x * kFrontTile + 4 * tx;
+    const int y0 = tile_y * kFrontTile + 4 * ty;
+    const uint32_t mask = (fc.bits >= 16) ? 0xffffu : ((1u << fc.bits) - 1u);
 
         float P[3][4][4];  // plane values, then linear R, G, B of the 4x4 block
 
@@ -98,12 +96,10 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, int njobs, const int* __
             const int64_t stride = job.stride[p];
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const int yy = min(y0 + j, H - 1);
+                const uint8_t* row = base + (int64_t)min(y0 + j, H - 1) * stride;
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int xx = min(x0 + i, W - 1);
-                    P[p][j][i] = apply_range(load_sample(base + yy * stride + (int64_t)xx * fc.bytes, fc.sample, mask), rx);
-                }
+                for (int i = 0; i < 4; i++)
+                    P[p][j][i] = apply_range(load_kind<KIND>(row + min(x0 + i, W - 1) * kBytes, mask), rx);
             }
         }
 
@@ -124,31 +120,33 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, int njobs, const int* __
             float* s_mid = lds + 2 * plane_elems;
             const RangeXform rx = make_range(fc.range, fc.bits, fc.yuv != 0);
             __syncthreads();  // previous item may still read LDS
-            for (int e = t; e < 2 * plane_elems; e += kFrontThreads) {
-                const int p = e / plane_elems;
-                const int rem = e - p * plane_elems;
-                const int r = rem / cs.ccols;
-                const int c = rem - r * cs.ccols;
-                const int gx = clampi(cs.cx0 + c, 0, cw - 1);
-                const int gy = clampi(cs.cy0 + r, 0, ch - 1);
-                s_chroma[e] = apply_range(
-                    load_sample(job.plane[1 + p] + gy * job.stride[1 + p] + (int64_t)gx * fc.bytes, fc.sample, mask), rx);
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                const uint8_t* base = job.plane[1 + p];
+                const int64_t stride = job.stride[1 + p];
+                for (int r = ty; r < cs.crows; r += 16) {
+                    const uint8_t* row = base + (int64_t)clampi(cs.cy0 + r, 0, ch - 1) * stride;
+                    for (int c = tx; c < cs.ccols; c += 16)
+                        s_chroma[p * plane_elems + r * cs.ccols + c] =
+                            apply_range(load_kind<KIND>(row + clampi(cs.cx0 + c, 0, cw - 1) * kBytes, mask), rx);
+                }
             }
             __syncthreads();
             if (cs.mid) {
                 // horizontal pass for every staged chroma row (reference runs
                 // it first, into a temporary plane, chromaUpsample.hpp:165-202)
                 const int mid_elems = cs.crows * kFrontTile;
-                for (int e = t; e < 2 * mid_elems; e += kFrontThreads) {
-                    const int p = e / mid_elems;
-                    const int rem = e - p * mid_elems;
-                    const int r = rem / kFrontTile;
-                    const int X = rem - r * kFrontTile;
-                    int k; float tt; bool copy;
-                    seg_of(tile_x * kFrontTile + X, fc.subw, fc.h_cosited != 0, k, tt, copy);
-                    const float* row = s_chroma + p * plane_elems + r * cs.ccols;
-                    const int c = k - cs.cx0;
-                    s_mid[e] = copy ? row[c] : catmull_rom(row[c - 1], row[c], row[c + 1], row[c + 2], tt);
+                const int X = t & (kFrontTile - 1);
+                int k; float tt; bool copy;
+                seg_of(tile_x * kFrontTile + X, fc.subw, fc.h_cosited != 0, k, tt, copy);
+                const int c = k - cs.cx0;
+#pragma unroll
+                for (int p = 0; p < 2; p++) {
+                    for (int r = t >> 6; r < cs.crows; r += kFrontThreads / kFrontTile) {
+                        const float* row = s_chroma + p * plane_elems + r * cs.ccols;
+                        s_mid[p * mid_elems + r * kFrontTile + X] =
+                            copy ? row[c] : catmull_rom(row[c - 1], row[c], row[c + 1], row[c + 2], tt);
+                    }
                 }
                 __syncthreads();
             }
@@ -368,6 +366,35 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, int njobs, const int* __
                 if (nb) out[geo.off[4][2] + o] = store_cast<StoreT>(q.b);
             }
         }
+}
+
+template <typename StoreT>
+__global__ __launch_bounds__(kFrontThreads)
+void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ counts,
+                     const FrontColor* __restrict__ colors, const PairGeom* __restrict__ geoms)
+{
+    extern __shared__ float lds[];
+    const int total = counts[kCountFrontTiles];
+    const int njobs = counts[kCountFrontJobs];
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        // job lookup: largest j with tile_start <= item
+        int lo = 0, hi = njobs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].tile_start <= item) lo = mid; else hi = mid - 1;
+        }
+        const FrontJob& job = jobs[lo];
+        const FrontColor fc = colors[job.color];
+        const PairGeom& geo = geoms[job.pair];
+        const int local = item - job.tile_start;
+        const int tile_x = local % job.tiles_x;
+        const int tile_y = local / job.tiles_x;
+        switch (fc.sample) {
+        case kSampleU8: front_tile<StoreT, kSampleU8>(job, fc, geo, tile_x, tile_y, lds); break;
+        case kSampleU16: front_tile<StoreT, kSampleU16>(job, fc, geo, tile_x, tile_y, lds); break;
+        case kSampleHalf: front_tile<StoreT, kSampleHalf>(job, fc, geo, tile_x, tile_y, lds); break;
+        default: front_tile<StoreT, kSampleFloat>(job, fc, geo, tile_x, tile_y, lds); break;
+        }
         __syncthreads();
     }
 }
@@ -386,17 +413,31 @@ size_t frontend_lds_bytes(const FrontColor& fc)
     return sizeof(float) * (chroma > levels ? chroma : levels);
 }
 
+size_t frontend_max_lds_bytes()
+{
+    size_t m = 0;
+    for (int subw = 0; subw <= 2; subw++)
+        for (int subh = 0; subh <= 1; subh++) {
+            FrontColor fc = {};
+            fc.subw = subw;
+            fc.subh = subh;
+            const size_t b = frontend_lds_bytes(fc);
+            m = b > m ? b : m;
+        }
+    return m;
+}
+
 template <typename StoreT>
-hipError_t launch_frontend(const FrontJob* jobs, int njobs, const int* total_tiles, int grid,
-                           const FrontColor* colors, const PairGeom* geoms, size_t lds_bytes, hipStream_t stream)
+hipError_t launch_frontend(const FrontJob* jobs, const int* counts, int grid, const FrontColor* colors,
+                           const PairGeom* geoms, size_t lds_bytes, hipStream_t stream)
 {
     if (grid <= 0) return hipSuccess;
     hipLaunchKernelGGL(frontend_kernel<StoreT>, dim3(grid), dim3(kFrontThreads), lds_bytes, stream,
-                       jobs, njobs, total_tiles, colors, geoms);
+                       jobs, counts, colors, geoms);
     return hipGetLastError();
 }
 
-template hipError_t launch_frontend<float>(const FrontJob*, int, const int*, int, const FrontColor*, const PairGeom*, size_t, hipStream_t);
-template hipError_t launch_frontend<__half>(const FrontJob*, int, const int*, int, const FrontColor*, const PairGeom*, size_t, hipStream_t);
+template hipError_t launch_frontend<float>(const FrontJob*, const int*, int, const FrontColor*, const PairGeom*, size_t, hipStream_t);
+template hipError_t launch_frontend<__half>(const FrontJob*, const int*, int, const FrontColor*, const PairGeom*, size_t, hipStream_t);
 
 }  // namespace vship

@@ -5,18 +5,37 @@
 
 namespace vship {
 
-// LDS bytes the front-end kernel needs for images of this colour description.
+// LDS bytes the front-end kernel needs for images of this colour description,
+// and the largest over every supported chroma subsampling.
 size_t frontend_lds_bytes(const FrontColor& fc);
+size_t frontend_max_lds_bytes();
 
+// counts: device array indexed by CountSlot. With fewer workgroups than items
+// the kernels loop over items (persistent mode).
 template <typename StoreT>
-hipError_t launch_frontend(const FrontJob* jobs, int njobs, const int* total_tiles, int grid,
-                           const FrontColor* colors, const PairGeom* geoms, size_t lds_bytes, hipStream_t stream);
+hipError_t launch_frontend(const FrontJob* jobs, const int* counts, int grid, const FrontColor* colors,
+                           const PairGeom* geoms, size_t lds_bytes, hipStream_t stream);
 
+// Blur/map tile shapes: config 0 = 64x32 (default), 1 = 64x16, 2 = 32x64,
+// 3 = 32x32, 4 = 64x64.
+struct BlurTile {
+    int tw, th;
+};
+BlurTile blur_tile_shape(int config);
+int blur_tile_configs();
+// Non-persistent grid covering total_tiles (a multiple of the XCD count).
+int blur_grid_for(int total_tiles);
+
+// grid must be a multiple of 8; with fewer workgroups than tiles the kernel
+// loops over tiles (persistent mode).
 template <typename StoreT>
-hipError_t launch_blur(const MapJob* jobs, int njobs, const int* total_tiles, int grid, const BlurTaps& taps,
+hipError_t launch_blur(int config, const MapJob* jobs, const int* counts, int grid, const BlurTaps& taps,
                        float* partials, hipStream_t stream);
 
-hipError_t launch_score(const ScoreJob* pairs, const int* npairs, int grid, const MapJob* jobs,
+hipError_t launch_score(const ScoreJob* pairs, const int* counts, int grid, const MapJob* jobs,
                         const float* partials, double* features, double* scores, hipStream_t stream);
+
+// Plain 16-byte-vector copy (bandwidth probe and counter calibration).
+hipError_t launch_copy(void* dst, const void* src, int64_t bytes, hipStream_t stream);
 
 }  // namespace vship

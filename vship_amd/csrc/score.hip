@@ -1,7 +1,7 @@
 // score.hip - FP64 completion of the per-scale norms and the final score.
 //
-// One workgroup per pair (grid-stride over pairs): for every active map job
-// of the pair it sums the per-tile float partials in double, divides by the
+// One workgroup per pair (grid-stride over pairs), one wave per active map job
+// of the pair: the wave sums the per-tile float partials in double, divides by the
 // scale's pixel count (L1 = mean, L4 = mean(x^4)^(1/4), reference
 // src/ssimu2/score.hpp:315-320,422-438), then forms the weighted sum of the
 // 108 features and the polynomial remap (score.hpp:442-460) in double. The
@@ -12,13 +12,15 @@
 #include "weights.hpp"
 #include "kernels.hpp"
 
+#include <algorithm>
+
 namespace vship {
 
 namespace {
 
 __constant__ double d_weights[108] = VSHIP_SSIMU2_WEIGHTS;
 
-constexpr int kScoreThreads = 256;
+constexpr int kScoreThreads = 1024;  // 16 waves: one wave per map job of a pair
 
 __device__ inline double wave_sum_d(double v)
 {
@@ -28,46 +30,41 @@ __device__ inline double wave_sum_d(double v)
 }
 
 __global__ __launch_bounds__(kScoreThreads)
-void score_kernel(const ScoreJob* __restrict__ pairs, const int* __restrict__ npairs_ptr,
+void score_kernel(const ScoreJob* __restrict__ pairs, const int* __restrict__ counts,
                   const MapJob* __restrict__ jobs, const float* __restrict__ partials,
                   double* __restrict__ features, double* __restrict__ scores)
 {
-    __shared__ double red[4][6];
     __shared__ double feat[108];
     const int t = threadIdx.x;
-    const int npairs = *npairs_ptr;
+    const int wave = t >> 6, lane = t & 63;
+    const int npairs = counts[kCountPairs];
     for (int pi = blockIdx.x; pi < npairs; pi += gridDim.x) {
         const ScoreJob pj = pairs[pi];
         for (int i = t; i < 108; i += kScoreThreads) feat[i] = 0.0;
         __syncthreads();
-        for (int j = pj.job_begin; j < pj.job_begin + pj.job_count; j++) {
-            const MapJob job = jobs[j];
+        for (int j = wave; j < pj.job_count; j += kScoreThreads / 64) {
+            const MapJob job = jobs[pj.job_begin + j];
             double acc[6] = {0, 0, 0, 0, 0, 0};
-            for (int k = t; k < job.ntiles; k += kScoreThreads) {
+            for (int k = lane; k < job.ntiles; k += 64) {
                 const float* p = partials + (int64_t)(job.tile_start + k) * kPartialStride;
 #pragma unroll
                 for (int m = 0; m < 6; m++) acc[m] += (double)p[m];
             }
 #pragma unroll
             for (int m = 0; m < 6; m++) acc[m] = wave_sum_d(acc[m]);
-            if ((t & 63) == 0) {
-#pragma unroll
-                for (int m = 0; m < 6; m++) red[t >> 6][m] = acc[m];
-            }
-            __syncthreads();
-            if (t < 3) {
-                const int m = t;
+            if (lane < 3) {
+                const int m = lane;
                 const int flag = (m == 0) ? kMapSsim : (m == 1 ? kMapArtifact : kMapDetail);
                 if (job.flags & flag) {
                     const double n = (double)job.w * (double)job.h;
-                    const double s1 = ((red[0][m] + red[1][m]) + red[2][m]) + red[3][m];
-                    const double s4 = ((red[0][3 + m] + red[1][3 + m]) + red[2][3 + m]) + red[3][3 + m];
+                    const double s1 = m == 0 ? acc[0] : (m == 1 ? acc[1] : acc[2]);
+                    const double s4 = m == 0 ? acc[3] : (m == 1 ? acc[4] : acc[5]);
                     feat[job.feature_base + m] = s1 / n;
                     feat[job.feature_base + 3 + m] = sqrt(sqrt(s4 / n));
                 }
             }
-            __syncthreads();
         }
+        __syncthreads();
         if (t == 0) {
             double s = 0.0;
             for (int i = 0; i < 108; i++) s = fma(d_weights[i], feat[i], s);
@@ -81,13 +78,29 @@ void score_kernel(const ScoreJob* __restrict__ pairs, const int* __restrict__ np
     }
 }
 
+__global__ __launch_bounds__(256) void copy_kernel(float4* __restrict__ dst, const float4* __restrict__ src, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace
 
-hipError_t launch_score(const ScoreJob* pairs, const int* npairs, int grid, const MapJob* jobs,
+hipError_t launch_copy(void* dst, const void* src, int64_t bytes, hipStream_t stream)
+{
+    const int64_t n = bytes / 16;
+    if (n <= 0) return hipSuccess;
+    const int grid = (int)std::min<int64_t>((n + 255) / 256, 256 * 16);
+    hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, stream, static_cast<float4*>(dst),
+                       static_cast<const float4*>(src), n);
+    return hipGetLastError();
+}
+
+hipError_t launch_score(const ScoreJob* pairs, const int* counts, int grid, const MapJob* jobs,
                         const float* partials, double* features, double* scores, hipStream_t stream)
 {
     if (grid <= 0) return hipSuccess;
-    hipLaunchKernelGGL(score_kernel, dim3(grid), dim3(kScoreThreads), 0, stream, pairs, npairs, jobs, partials,
+    hipLaunchKernelGGL(score_kernel, dim3(grid), dim3(kScoreThreads), 0, stream, pairs, counts, jobs, partials,
                        features, scores);
     return hipGetLastError();
 }

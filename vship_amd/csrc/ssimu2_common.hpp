@@ -24,15 +24,25 @@ constexpr float kC2 = 0.0009f;    // SSIM constant (score.hpp:272-273)
 constexpr int kFrontTile = 64;
 constexpr int kFrontThreads = 256;
 
-// Blur/map tile: output kBlurTW x kBlurTH, halo 8 on every side.
-constexpr int kBlurTW = 64;
-constexpr int kBlurTH = 32;
+// Blur/map tiles: 256 threads, output tile shape chosen per launch (blur.hip),
+// halo 8 on every side.
 constexpr int kBlurThreads = 256;
 constexpr int kHalo = 8;
 
 // Partial sums written per blur tile: {sum ssim, sum art, sum det,
 // sum ssim^4, sum art^4, sum det^4}, padded to 8 floats.
 constexpr int kPartialStride = 8;
+
+// Device-resident item counts read by the persistent kernels, so that one
+// captured launch sequence serves any batch composition.
+enum CountSlot : int32_t {
+    kCountFrontTiles = 0,
+    kCountFrontJobs = 1,
+    kCountBlurTiles = 2,
+    kCountBlurJobs = 3,
+    kCountPairs = 4,
+    kCountSlots = 8,
+};
 
 enum MapFlags : int32_t {
     kMapSsim = 1,

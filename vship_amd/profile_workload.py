@@ -1,0 +1,83 @@
+"""Fixed workloads for profiling the SSIMULACRA2 kernels (rocprofv3 targets).
+
+python -m vship_amd.profile_workload plain --pairs 8     # plain FP32 path, 1080p YUV420
+python -m vship_amd.profile_workload batch --pairs 240   # batched FP16 path, 1080p YUV420
+python -m vship_amd.profile_workload copy --mib 256      # plain copy kernel (counter calibration)
+
+Each mode prints what it ran and the HIP-event time per kernel.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import sys
+
+import numpy as np
+
+from . import ssimu2 as S
+from . import synth
+
+
+def _upload_pairs(n: int, seed: int = 0):
+    seq = synth.YuvSequence(1920, 1080, n, seed=seed)
+    bufs, ptrs = [], []
+    strides = None
+    for i in range(n):
+        ref, dist, _ = seq.frame(i)
+        strides = S.host_strides(ref)
+        pair = []
+        for img in (ref, dist):
+            trip = []
+            for p in img:
+                b = S.DeviceBuffer(p.nbytes)
+                b.upload(p)
+                bufs.append(b)
+                trip.append(b.address)
+            pair.append(trip)
+        ptrs.append(pair)
+    return bufs, ptrs, strides
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("mode", choices=["plain", "batch", "copy"])
+    ap.add_argument("--pairs", type=int, default=8)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--mib", type=int, default=256)
+    ap.add_argument("--storage", choices=["fp16", "fp32"], default="fp16")
+    a = ap.parse_args(argv)
+    if a.mode == "copy":
+        nbytes = a.mib << 20
+        src, dst = S.DeviceBuffer(nbytes), S.DeviceBuffer(nbytes)
+        ms = ctypes.c_double()
+        S.check(S.load_library().VshipAMD_DeviceCopy(ctypes.c_void_p(dst.address), ctypes.c_void_p(src.address),
+                                                       nbytes, a.reps, ctypes.byref(ms)))
+        print(f"copy {nbytes} bytes: {ms.value:.4f} ms, {2 * nbytes / ms.value / 1e6:.1f} GB/s read+write", flush=True)
+        return 0
+    bufs, ptrs, st = _upload_pairs(a.pairs)
+    cs = S.colorspace(1920, 1080)
+    print(f"{a.mode}: {a.pairs} pairs 1920x1080 yuv420p8, {a.reps} reps", flush=True)
+    if a.mode == "plain":
+        with S.SSIMU2Handler(cs, cs) as h:
+            for _ in range(a.reps):
+                scores = [h.compute_device(p[0], p[1], st, st) for p in ptrs]
+            fr, bl, sc = h.profile(a.reps)
+        print(f"scores mean {np.mean(scores):.5f}; per pair ms frontend {fr:.4f} blur {bl:.4f} score {sc:.4f}",
+              flush=True)
+    else:
+        storage = S.STORAGE_FP16 if a.storage == "fp16" else S.STORAGE_FP32
+        with S.SSIMU2Batch(a.pairs, storage) as b:
+            args = ([cs] * a.pairs, [cs] * a.pairs, [p[0] for p in ptrs], [p[1] for p in ptrs], [st] * a.pairs,
+                    [st] * a.pairs)
+            for _ in range(a.reps):
+                scores = b.compute(*args)
+            fr, bl, sc, gr = b.profile(a.reps)
+            print(f"tuning {b.tuning()}; scores mean {np.mean(scores):.5f}; per batch ms frontend {fr:.4f} "
+                  f"blur {bl:.4f} score {sc:.4f} graph {gr:.4f}", flush=True)
+    for b in bufs:
+        b.free()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

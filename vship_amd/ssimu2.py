@@ -100,6 +100,14 @@ class Handler(ctypes.Structure):
     _fields_ = [("id", ctypes.c_uint)]
 
 
+class BatchTuning(ctypes.Structure):
+    """VshipAMD_BatchTuning: blur tile shape id, blur and front-end WGs per CU."""
+    _fields_ = [("blur_config", ctypes.c_int), ("blur_waves", ctypes.c_int), ("front_waves", ctypes.c_int)]
+
+    def as_dict(self):
+        return {"blur_config": self.blur_config, "blur_waves": self.blur_waves, "front_waves": self.front_waves}
+
+
 class Version(ctypes.Structure):
     _fields_ = [("major", ctypes.c_int), ("minor", ctypes.c_int), ("minorMinor", ctypes.c_int),
                 ("backend", ctypes.c_int)]
@@ -155,11 +163,25 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "VshipAMD_SSIMU2Profile": (E, [Handler, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double),
                                        P(ctypes.c_double)]),
         "VshipAMD_SSIMU2Work": (E, [Handler, P(ctypes.c_double), P(ctypes.c_double)]),
+        "VshipAMD_SSIMU2BatchInit": (E, [P(Handler), ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "VshipAMD_SSIMU2BatchFree": (E, [Handler]),
+        "VshipAMD_SSIMU2BatchCompute": (E, [Handler, ctypes.c_int, P(Colorspace), P(Colorspace), u8pp, u8pp, i64p,
+                                            i64p, P(ctypes.c_double)]),
+        "VshipAMD_SSIMU2BatchAutotune": (E, [Handler, ctypes.c_int, P(Colorspace), P(Colorspace), u8pp, u8pp, i64p,
+                                             i64p, ctypes.c_int, P(BatchTuning), P(ctypes.c_double)]),
+        "VshipAMD_SSIMU2BatchSetTuning": (E, [Handler, BatchTuning]),
+        "VshipAMD_SSIMU2BatchGetTuning": (E, [Handler, P(BatchTuning)]),
+        "VshipAMD_SSIMU2BatchFeatures": (E, [Handler, P(ctypes.c_double)]),
+        "VshipAMD_SSIMU2BatchProfile": (E, [Handler, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double),
+                                            P(ctypes.c_double), P(ctypes.c_double)]),
+        "VshipAMD_SSIMU2BatchWork": (E, [Handler, P(ctypes.c_double), P(ctypes.c_double)]),
         "VshipAMD_DeviceMalloc": (E, [P(ctypes.c_void_p), ctypes.c_uint64, ctypes.c_int]),
         "VshipAMD_DeviceFree": (E, [ctypes.c_void_p]),
         "VshipAMD_MemcpyHtoD": (E, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]),
         "VshipAMD_MemcpyDtoH": (E, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]),
         "VshipAMD_DeviceSynchronize": (E, []),
+        "VshipAMD_DeviceCopy": (E, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
+                                    P(ctypes.c_double)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -321,6 +343,87 @@ class SSIMU2Handler:
         if getattr(self, "_open", False):
             self._open = False
             check(self._lib.Vship_SSIMU2Free(self.handle))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+STORAGE_FP32, STORAGE_FP16 = 0, 1
+
+
+class SSIMU2Batch:
+    """A VshipAMD_SSIMU2Batch context: many device-resident pairs per call."""
+
+    def __init__(self, max_pairs: int, storage: int = STORAGE_FP16, gpu_id: int = 0):
+        self._lib = load_library()
+        self.handle = Handler()
+        check(self._lib.VshipAMD_SSIMU2BatchInit(ctypes.byref(self.handle), max_pairs, storage, gpu_id))
+        self._open = True
+        self.max_pairs = max_pairs
+
+    @staticmethod
+    def _args(src: Sequence[Colorspace], dis: Sequence[Colorspace], planes1, planes2, strides1, strides2):
+        n = len(src)
+        cs1 = (Colorspace * n)(*src)
+        cs2 = (Colorspace * n)(*dis)
+        p1 = (ctypes.c_void_p * (3 * n))(*[int(x) for trip in planes1 for x in trip])
+        p2 = (ctypes.c_void_p * (3 * n))(*[int(x) for trip in planes2 for x in trip])
+        l1 = (ctypes.c_int64 * (3 * n))(*[int(x) for trip in strides1 for x in trip])
+        l2 = (ctypes.c_int64 * (3 * n))(*[int(x) for trip in strides2 for x in trip])
+        return n, cs1, cs2, p1, p2, l1, l2
+
+    def compute(self, src, dis, planes1, planes2, strides1, strides2) -> np.ndarray:
+        """Scores of len(src) pairs; planesK[i] are 3 device pointers, stridesK[i] 3 line sizes."""
+        n, cs1, cs2, p1, p2, l1, l2 = self._args(src, dis, planes1, planes2, strides1, strides2)
+        out = np.zeros(n, dtype=np.float64)
+        check(self._lib.VshipAMD_SSIMU2BatchCompute(self.handle, n, cs1, cs2, p1, p2, l1, l2,
+                                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def autotune(self, src, dis, planes1, planes2, strides1, strides2, reps: int = 2):
+        n, cs1, cs2, p1, p2, l1, l2 = self._args(src, dis, planes1, planes2, strides1, strides2)
+        t = BatchTuning()
+        ms = ctypes.c_double()
+        check(self._lib.VshipAMD_SSIMU2BatchAutotune(self.handle, n, cs1, cs2, p1, p2, l1, l2, reps,
+                                                      ctypes.byref(t), ctypes.byref(ms)))
+        return t.as_dict(), ms.value
+
+    def set_tuning(self, blur_config: int, blur_waves: int, front_waves: int) -> None:
+        check(self._lib.VshipAMD_SSIMU2BatchSetTuning(self.handle, BatchTuning(blur_config, blur_waves, front_waves)))
+
+    def tuning(self) -> dict:
+        t = BatchTuning()
+        check(self._lib.VshipAMD_SSIMU2BatchGetTuning(self.handle, ctypes.byref(t)))
+        return t.as_dict()
+
+    def features(self, npairs: int) -> np.ndarray:
+        out = np.zeros((npairs, 108), dtype=np.float64)
+        check(self._lib.VshipAMD_SSIMU2BatchFeatures(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def profile(self, iters: int = 5):
+        v = [ctypes.c_double() for _ in range(4)]
+        check(self._lib.VshipAMD_SSIMU2BatchProfile(self.handle, iters, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def work(self):
+        b, f = ctypes.c_double(), ctypes.c_double()
+        check(self._lib.VshipAMD_SSIMU2BatchWork(self.handle, ctypes.byref(b), ctypes.byref(f)))
+        return b.value, f.value
+
+    def close(self) -> None:
+        if getattr(self, "_open", False):
+            self._open = False
+            check(self._lib.VshipAMD_SSIMU2BatchFree(self.handle))
 
     def __enter__(self):
         return self
