@@ -200,7 +200,12 @@ def test_against_reference_libvship():
         _say(f"ref {name}: gpu {got:.6f} reference {want_ref:.6f} oracle {want_or:.6f} "
              f"gpu-oracle {got - want_or:+.2e} ref-oracle {want_ref - want_or:+.2e}")
         assert abs(got - want_or) <= SCORE_TOL, name
-        if name.startswith("random"):
+        if name.endswith("identity"):
+            # the reference's SSIM numerator and denominator round differently
+            # for a == b, so it scores identical frames below 100; its own
+            # identity test expects exactly 100, which this library returns
+            assert got == 100.0 and want_or == 100.0
+        elif name.startswith("random"):
             assert abs(got - want_ref) <= abs(want_ref - want_or) + SCORE_TOL, name
         else:
             assert abs(got - want_ref) <= SCORE_TOL, name

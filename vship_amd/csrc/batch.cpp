@@ -30,12 +30,12 @@ Ssimu2Batch::Ssimu2Batch(int max_pairs, bool fp16, int device)
     VSHIP_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 
     // fixed-capacity packed tables: counts | front jobs | colours | geoms | maps | score jobs
-    cap_fjobs_ = 2 * max_pairs;
+    cap_fjobs_ = max_pairs;
     cap_maps_ = kMaxJobsPerPair * max_pairs;
     size_t off = 0;
     const size_t o_counts = off; off = align_up(off + kCountSlots * sizeof(int), 256);
     const size_t o_fjobs = off; off = align_up(off + cap_fjobs_ * sizeof(FrontJob), 256);
-    const size_t o_colors = off; off = align_up(off + cap_fjobs_ * sizeof(FrontColor), 256);
+    const size_t o_colors = off; off = align_up(off + 2 * cap_fjobs_ * sizeof(FrontColor), 256);
     const size_t o_geoms = off; off = align_up(off + max_pairs * sizeof(PairGeom), 256);
     const size_t o_maps = off; off = align_up(off + cap_maps_ * sizeof(MapJob), 256);
     const size_t o_sjobs = off; off = align_up(off + max_pairs * sizeof(ScoreJob), 256);
@@ -97,34 +97,34 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
         for (int k = 0; k < 2; k++)
             for (int p = 0; p < 3; p++) {
                 if (pl[k][p] == nullptr) VSHIP_THROW(Vship_BadPointer, "null plane pointer");
-                if (ls[k][p] < specs[k]->plane_w[p] * specs[k]->fc.bytes)
+                if (ls[k][p] < specs[k]->plane_w[p] * specs[k]->bytes)
                     VSHIP_THROW(Vship_BadPointer, "lineSize smaller than one row of samples");
-                in_bytes += (double)specs[k]->plane_w[p] * specs[k]->plane_h[p] * specs[k]->fc.bytes;
+                in_bytes += (double)specs[k]->plane_w[p] * specs[k]->plane_h[p] * specs[k]->bytes;
             }
         layouts.push_back(make_pair_layout((int)s1.width, (int)s1.height, tuning_.blur_config));
         const PairLayout& L = layouts.back();
         geoms_.push_back(L.geom);
+        FrontJob j;
+        std::memset(&j, 0, sizeof(j));
         for (int k = 0; k < 2; k++) {
             // keep every image buffer 256-byte aligned
             elems = (int64_t)align_up((size_t)elems, 256 / elem);
             xyb_off_.push_back(elems);
             elems += L.geom.image_elems;
-            FrontJob j;
-            std::memset(&j, 0, sizeof(j));
             for (int p = 0; p < 3; p++) {
-                j.plane[p] = pl[k][p];
-                j.stride[p] = ls[k][p];
+                j.plane[k][p] = pl[k][p];
+                j.stride[k][p] = ls[k][p];
             }
-            j.width = (int32_t)s1.width;
-            j.height = (int32_t)s1.height;
-            j.tiles_x = L.front_tiles_x;
-            j.tile_start = ftiles;
-            j.pair = i;
-            j.color = (int32_t)colors_.size();
+            j.color[k] = (int32_t)colors_.size();
             colors_.push_back(specs[k]->fc);
-            ftiles += L.front_tiles_per_image;
-            fjobs_.push_back(j);
         }
+        j.width = (int32_t)s1.width;
+        j.height = (int32_t)s1.height;
+        j.tiles_x = L.front_tiles_x;
+        j.tile_start = ftiles;
+        j.pair = i;
+        ftiles += L.front_tiles_per_image;
+        fjobs_.push_back(j);
         ScoreJob sj = {(int32_t)maps_.size(), (int32_t)L.maps.size(), {0, 0}};
         sjobs_.push_back(sj);
         for (const MapJob& m0 : L.maps) {
@@ -176,10 +176,8 @@ void Ssimu2Batch::ensure_capacity()
         graph_valid_ = false;  // the kernels take the partials pointer as an argument
     }
     // final pointers into the arena, then one packed upload
-    for (auto& j : fjobs_) {
-        const int64_t off = xyb_off_[&j - fjobs_.data()];
-        j.out = static_cast<uint8_t*>(xyb_) + off * elem;
-    }
+    for (size_t i = 0; i < fjobs_.size(); i++)
+        for (int k = 0; k < 2; k++) fjobs_[i].out[k] = static_cast<uint8_t*>(xyb_) + xyb_off_[2 * i + k] * elem;
     std::vector<MapJob> dev_maps = maps_;
     for (auto& m : dev_maps) {
         m.a = static_cast<const uint8_t*>(xyb_) + reinterpret_cast<int64_t>(m.a) * elem;

@@ -110,8 +110,11 @@ __device__ void blur_tile(const MapJob& job, int tile_x, int tile_y, const BlurT
             load4<StoreT>(B + o, vb);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
+                // FP16 planes hold the distorted image as its difference
+                // from the reference: b = a + d
+                const float bv = (sizeof(StoreT) == 2) ? va[i] + vb[i] : vb[i];
                 va[i] = (gx + i < w) ? va[i] - c : 0.f;
-                vb[i] = (gx + i < w) ? vb[i] - c : 0.f;
+                vb[i] = (gx + i < w) ? bv - c : 0.f;
             }
         }
         *reinterpret_cast<float4*>(sA + r * Sh::SW + 4 * c4) = make_float4(va[0], va[1], va[2], va[3]);

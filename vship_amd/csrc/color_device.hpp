@@ -15,28 +15,6 @@ namespace vship {
 
 enum SampleKind : int32_t { kSampleU8 = 0, kSampleU16 = 1, kSampleHalf = 2, kSampleFloat = 3 };
 
-// reference src/gpuColorToLinear/anyDepthToFloat.hpp:24-68 (PickValue):
-// little endian, integer samples above 8 bits masked to their bit depth.
-__device__ inline float load_sample(const uint8_t* p, int kind, uint32_t mask)
-{
-    switch (kind) {
-    case kSampleU8:
-        return (float)p[0];
-    case kSampleU16: {
-        const uint32_t v = (uint32_t)p[0] | ((uint32_t)p[1] << 8);
-        return (float)(v & mask);
-    }
-    case kSampleHalf: {
-        const uint16_t v = (uint16_t)((uint32_t)p[0] | ((uint32_t)p[1] << 8));
-        return __half2float(__ushort_as_half(v));
-    }
-    default: {
-        const uint32_t v = (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
-        return __uint_as_float(v);
-    }
-    }
-}
-
 // Range normalisation constants of one plane, reference
 // src/gpuColorToLinear/rangeToFull.hpp:24-55: v' = (v * scale - offset) * gain
 // full range:    scale = 1/(2^b - 1), offset = 0.5 (YUV chroma) or 0, gain = 1
@@ -144,60 +122,13 @@ __device__ inline void eotf_hlg(float& r, float& g, float& b)
     r = v[0] * k; g = v[1] * k; b = v[2] * k;
 }
 
-// Y'CbCr (y = Y, u = Cb, v = Cr) -> linear RGB, reference
-// src/gpuColorToLinear/YUVToLinRGB.hpp:157-163 (gamma-light matrix, EOTF,
-// linear-light matrix; per-matrix steps :7-155).
-__device__ inline void to_linear_rgb(float y, float u, float v, int matrix, int trc, float& R, float& G, float& B)
+// 3x3 row-major matrix times (x, y, z), in place.
+__device__ inline void mat3_apply(const float m[9], float& x, float& y, float& z)
 {
-    float r, g, b;
-    switch (matrix) {
-    case 1: case 5: case 6: case 9: {
-        float kr, kb;
-        if (matrix == 1) { kr = 0.2126f; kb = 0.0722f; }
-        else if (matrix == 9) { kr = 0.2627f; kb = 0.0593f; }
-        else { kr = 0.299f; kb = 0.114f; }
-        const float kg = 1.0f - kb - kr;
-        const float us = 0.5f / (1.0f - kb), vs = 0.5f / (1.0f - kr);
-        r = (v + y * vs) * (1.0f / (vs * kr + 0.5f));
-        b = (u + y * us) * (1.0f / (us * kb + 0.5f));
-        g = (y - kr * r - kb * b) * (1.0f / kg);
-        break;
-    }
-    case 10: { /* BT2020 CL: {R', Y', B'} */
-        const float kr = 0.2627f, kb = 0.0593f;
-        const float us = 0.5f / (1.0f - kb), vs = 0.5f / (1.0f - kr);
-        r = (v + y * vs) * (1.0f / (vs * kr + 0.5f));
-        b = (u + y * us) * (1.0f / (us * kb + 0.5f));
-        g = y;
-        break;
-    }
-    case 14: { /* ICtCp -> L'M'S' */
-        r = fmaf(u, 0.0086090370379327566f, fmaf(v, 0.11102962500302595655f, y));
-        g = fmaf(u, -0.01721807407586551f, fmaf(v, -0.22205925000605192f, r));
-        b = fmaf(r, 33.025782688766114f, fmaf(g, -32.025782688766114f, -v * 7.543278084714549f));
-        break;
-    }
-    default: /* RGB */
-        r = y; g = u; b = v;
-        break;
-    }
-    if (trc == 18) {
-        eotf_hlg(r, g, b);
-    } else {
-        r = eotf(r, trc);
-        g = eotf(g, trc);
-        b = eotf(b, trc);
-    }
-    if (matrix == 10) {
-        const float kr = 0.2627f, kb = 0.0593f, kg = 1.0f - kr - kb;
-        g = (g - kr * r - kb * b) * (1.0f / kg);
-    } else if (matrix == 14) {
-        const float l = r, m = g, s = b;
-        r = fmaf(l, 3.4366066943330784267f, fmaf(m, -2.5064521186562698975f, s * 0.069845424323191470954f));
-        g = fmaf(r, -0.23026480071281397f, fmaf(m, 1.4064527541936944f, -s * 0.2083538389058436f));
-        b = fmaf(r, -0.02684381778741865f, fmaf(g, -0.08378524945770065f, s * 1.106879231557486f));
-    }
-    R = r; G = g; B = b;
+    const float a = fmaf(m[0], x, fmaf(m[1], y, m[2] * z));
+    const float b = fmaf(m[3], x, fmaf(m[4], y, m[5] * z));
+    const float c = fmaf(m[6], x, fmaf(m[7], y, m[8] * z));
+    x = a; y = b; z = c;
 }
 
 // Linear RGB -> positive XYB, reference src/ssimu2/makeXYB.hpp:6-52. The last

@@ -7,6 +7,7 @@
 
 #include "errors.hpp"
 #include "kernels.hpp"
+#include "color_device.hpp"
 #include "weights.hpp"
 
 namespace vship {
@@ -52,30 +53,103 @@ void matmul3(const double a[9], const double b[9], double out[9])
         for (int c = 0; c < 3; c++) out[r * 3 + c] = a[r * 3 + 0] * b[0 * 3 + c] + a[r * 3 + 1] * b[1 * 3 + c] + a[r * 3 + 2] * b[2 * 3 + c];
 }
 
-// Source primaries -> BT.709 as one matrix (primaries.hpp:134-144,195-220).
-bool primaries_matrix(int prim, float out[9])
+// Source primaries -> BT.709 as one matrix (primaries.hpp:134-144,195-220);
+// identity for BT.709.
+void primaries_matrix(int prim, double out[9])
 {
     const double* to = nullptr;
-    double m[9];
     switch (prim) {
-    case Vship_PRIMARIES_BT709: return false;
     case Vship_PRIMARIES_BT2020:
-        for (int i = 0; i < 9; i++) out[i] = (float)k2020To709[i];
-        return true;
+        for (int i = 0; i < 9; i++) out[i] = k2020To709[i];
+        return;
     case Vship_PRIMARIES_INTERNAL:
-        for (int i = 0; i < 9; i++) out[i] = (float)kFromXyz709[i];
-        return true;
+        for (int i = 0; i < 9; i++) out[i] = kFromXyz709[i];
+        return;
     case Vship_PRIMARIES_BT470_M: to = kToXyz470M; break;
     case Vship_PRIMARIES_BT470_BG: to = kToXyz470BG; break;
     case Vship_PRIMARIES_ST170_M:
     case Vship_PRIMARIES_ST240_M: to = kToXyz170M; break;
-    default: return false;
+    default:
+        for (int i = 0; i < 9; i++) out[i] = (i % 4 == 0) ? 1.0 : 0.0;
+        return;
     }
     (void)kToXyz709;
     (void)kToXyz2020;
-    matmul3(kFromXyz709, to, m);
-    for (int i = 0; i < 9; i++) out[i] = (float)m[i];
-    return true;
+    matmul3(kFromXyz709, to, out);
+}
+
+// Rows of R' and B' from (Y', Cb, Cr) for constants Kr, Kb, written as the
+// reference evaluates them (YUVToLinRGB.hpp:7-18): R = (Cr + Y vs)/(vs Kr + 1/2),
+// B = (Cb + Y us)/(us Kb + 1/2).
+void ncl_rows(double kr, double kb, double R[3], double B[3])
+{
+    const double us = 0.5 / (1.0 - kb), vs = 0.5 / (1.0 - kr);
+    const double dr = vs * kr + 0.5, db = us * kb + 0.5;
+    R[0] = vs / dr; R[1] = 0.0; R[2] = 1.0 / dr;
+    B[0] = us / db; B[1] = 1.0 / db; B[2] = 0.0;
+}
+
+// The matrix steps of YUVToLinRGB.hpp:157-163 as pre (before the EOTF) and
+// post (after it) 3x3 matrices; post also carries the primaries conversion.
+void colour_matrices(int matrix, int prim, double pre[9], double post[9], bool* use_pre)
+{
+    double id[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    double lin[9];
+    for (int i = 0; i < 9; i++) { pre[i] = id[i]; lin[i] = id[i]; }
+    *use_pre = true;
+    switch (matrix) {
+    case Vship_MATRIX_BT709: case Vship_MATRIX_BT470_BG: case Vship_MATRIX_ST170_M: case Vship_MATRIX_BT2020_NCL: {
+        double kr = 0.2126, kb = 0.0722;
+        if (matrix == Vship_MATRIX_BT470_BG || matrix == Vship_MATRIX_ST170_M) { kr = 0.299; kb = 0.114; }
+        if (matrix == Vship_MATRIX_BT2020_NCL) { kr = 0.2627; kb = 0.0593; }
+        const double kg = 1.0 - kb - kr;
+        double R[3], B[3];
+        ncl_rows(kr, kb, R, B);
+        for (int c = 0; c < 3; c++) {
+            pre[0 * 3 + c] = R[c];
+            pre[2 * 3 + c] = B[c];
+            pre[1 * 3 + c] = ((c == 0 ? 1.0 : 0.0) - kr * R[c] - kb * B[c]) / kg;
+        }
+        break;
+    }
+    case Vship_MATRIX_BT2020_CL: {
+        // YUVToLinRGB.hpp:65-94: {R', Y', B'}, EOTF, then G from the linear luminance
+        const double kr = 0.2627, kb = 0.0593, kg = 1.0 - kr - kb;
+        double R[3], B[3];
+        ncl_rows(kr, kb, R, B);
+        for (int c = 0; c < 3; c++) {
+            pre[0 * 3 + c] = R[c];
+            pre[1 * 3 + c] = (c == 0) ? 1.0 : 0.0;
+            pre[2 * 3 + c] = B[c];
+        }
+        const double g[9] = {1, 0, 0, -kr / kg, 1.0 / kg, -kb / kg, 0, 0, 1};
+        for (int i = 0; i < 9; i++) lin[i] = g[i];
+        break;
+    }
+    case Vship_MATRIX_BT2100_ICTCP: {
+        // YUVToLinRGB.hpp:26-63: ICtCp -> L'M'S' (rows expanded), EOTF, LMS -> RGB
+        const double L[3] = {1.0, 0.0086090370379327566, 0.11102962500302595655};
+        const double M[3] = {L[0], L[1] - 0.01721807407586551, L[2] - 0.22205925000605192};
+        double S3[3];
+        for (int c = 0; c < 3; c++)
+            S3[c] = 33.025782688766114 * L[c] - 32.025782688766114 * M[c] - (c == 2 ? 7.543278084714549 : 0.0);
+        for (int c = 0; c < 3; c++) { pre[c] = L[c]; pre[3 + c] = M[c]; pre[6 + c] = S3[c]; }
+        const double Rr[3] = {3.4366066943330784267, -2.5064521186562698975, 0.069845424323191470954};
+        double Gr[3], Br[3];
+        for (int c = 0; c < 3; c++)
+            Gr[c] = -0.23026480071281397 * Rr[c] + (c == 1 ? 1.4064527541936944 : 0.0) + (c == 2 ? -0.2083538389058436 : 0.0);
+        for (int c = 0; c < 3; c++)
+            Br[c] = -0.02684381778741865 * Rr[c] - 0.08378524945770065 * Gr[c] + (c == 2 ? 1.106879231557486 : 0.0);
+        for (int c = 0; c < 3; c++) { lin[c] = Rr[c]; lin[3 + c] = Gr[c]; lin[6 + c] = Br[c]; }
+        break;
+    }
+    default:
+        *use_pre = false;
+        break;
+    }
+    double prm[9];
+    primaries_matrix(prim, prm);
+    matmul3(prm, lin, post);
 }
 
 int64_t half_up(int64_t n) { return (n - 1) / 2 + 1; }
@@ -127,18 +201,33 @@ ImageSpec make_image_spec(const Vship_Colorspace_t& cs)
         VSHIP_THROW(Vship_NonRGBSInput, "crop and resize are not supported by this library");
 
     FrontColor& fc = s.fc;
+    const int bits = sample_bits(cs.sample);
     fc.sample = cs.sample == Vship_SampleUINT8 ? 0 : (cs.sample == Vship_SampleHALF ? 2 : (cs.sample == Vship_SampleFLOAT ? 3 : 1));
-    fc.bits = sample_bits(cs.sample);
-    fc.bytes = cs.sample == Vship_SampleUINT8 ? 1 : (cs.sample == Vship_SampleFLOAT ? 4 : 2);
-    fc.range = cs.range;
-    fc.yuv = cs.colorFamily == Vship_ColorYUV;
+    fc.mask = bits >= 16 ? 0xffffu : ((1u << bits) - 1u);
+    s.bytes = cs.sample == Vship_SampleUINT8 ? 1 : (cs.sample == Vship_SampleFLOAT ? 4 : 2);
     fc.subw = cs.subsampling.subw;
     fc.subh = cs.subsampling.subh;
     fc.h_cosited = (cs.chromaLocation == Vship_ChromaLoc_Left || cs.chromaLocation == Vship_ChromaLoc_TopLeft);
     fc.v_cosited = (cs.chromaLocation == Vship_ChromaLoc_Top || cs.chromaLocation == Vship_ChromaLoc_TopLeft);
-    fc.matrix = cs.YUVMatrix;
     fc.transfer = cs.transferFunction;
-    fc.use_prim = primaries_matrix(cs.primaries, fc.prim) ? 1 : 0;
+    const RangeXform r0 = make_range(cs.range, bits, false);
+    const RangeXform r12 = make_range(cs.range, bits, cs.colorFamily == Vship_ColorYUV);
+    fc.range0[0] = r0.scale; fc.range0[1] = r0.offset; fc.range0[2] = r0.gain;
+    fc.range12[0] = r12.scale; fc.range12[1] = r12.offset; fc.range12[2] = r12.gain;
+    // the reference skips the whole matrix/EOTF step for linear RGB (YUVToLinRGB.hpp:219)
+    const int matrix = (cs.transferFunction == Vship_TRC_Linear && cs.YUVMatrix == Vship_MATRIX_RGB) ? Vship_MATRIX_RGB
+                                                                                                      : cs.YUVMatrix;
+    double pre[9], post[9];
+    bool use_pre = false;
+    colour_matrices(matrix, cs.primaries, pre, post, &use_pre);
+    fc.use_pre = use_pre ? 1 : 0;
+    bool ident = true;
+    for (int i = 0; i < 9; i++) {
+        fc.pre[i] = (float)pre[i];
+        fc.post[i] = (float)post[i];
+        ident = ident && post[i] == ((i % 4 == 0) ? 1.0 : 0.0);
+    }
+    fc.use_post = ident ? 0 : 1;
     s.width = cs.width;
     s.height = cs.height;
     s.plane_w[0] = cs.width;
@@ -187,6 +276,10 @@ PairLayout make_pair_layout(int w, int h, int blur_config)
         ch = half_up(ch);
     }
     L.geom.image_elems = off;
+    for (int sc = 0; sc < kUsedScales; sc++)
+        for (int p = 0; p < 3; p++)
+            if ((L.geom.off[sc][p] >= 0) != kStored[sc][p])
+                VSHIP_THROW(Vship_BadErrorType, "stored plane set differs from the kernels' table");
     L.front_tiles_x = (w + kFrontTile - 1) / kFrontTile;
     L.front_tiles_per_image = L.front_tiles_x * ((h + kFrontTile - 1) / kFrontTile);
     int tiles = 0;
@@ -236,7 +329,7 @@ Ssimu2Handler::Ssimu2Handler(const Vship_Colorspace_t& src, const Vship_Colorspa
     };
     const size_t elems = (size_t)layout_.geom.image_elems;
     for (int i = 0; i < 2; i++) dmalloc(&xyb_[i], elems * sizeof(float));
-    dmalloc((void**)&d_front_jobs_, 2 * sizeof(FrontJob));
+    dmalloc((void**)&d_front_jobs_, sizeof(FrontJob));
     dmalloc((void**)&d_colors_, 2 * sizeof(FrontColor));
     dmalloc((void**)&d_geom_, sizeof(PairGeom));
     dmalloc((void**)&d_maps_, layout_.maps.size() * sizeof(MapJob));
@@ -245,7 +338,7 @@ Ssimu2Handler::Ssimu2Handler(const Vship_Colorspace_t& src, const Vship_Colorspa
     dmalloc((void**)&d_partials_, (size_t)layout_.blur_tiles * kPartialStride * sizeof(float));
     dmalloc((void**)&d_features_, 108 * sizeof(double));
     dmalloc((void**)&d_score_, sizeof(double));
-    if (hipHostMalloc((void**)&h_front_jobs_, 2 * sizeof(FrontJob), hipHostMallocDefault) != hipSuccess ||
+    if (hipHostMalloc((void**)&h_front_jobs_, sizeof(FrontJob), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&h_score_, sizeof(double), hipHostMallocDefault) != hipSuccess)
         VSHIP_THROW(Vship_OutOfRAM, "hipHostMalloc failed in SSIMU2 init");
 
@@ -259,8 +352,8 @@ Ssimu2Handler::Ssimu2Handler(const Vship_Colorspace_t& src, const Vship_Colorspa
     FrontColor colors[2] = {spec_[0].fc, spec_[1].fc};
     ScoreJob sj = {0, (int32_t)maps.size(), {0, 0}};
     int counts[kCountSlots] = {};
-    counts[kCountFrontTiles] = 2 * layout_.front_tiles_per_image;
-    counts[kCountFrontJobs] = 2;
+    counts[kCountFrontTiles] = layout_.front_tiles_per_image;
+    counts[kCountFrontJobs] = 1;
     counts[kCountBlurTiles] = layout_.blur_tiles;
     counts[kCountBlurJobs] = (int)maps.size();
     counts[kCountPairs] = 1;
@@ -270,17 +363,17 @@ Ssimu2Handler::Ssimu2Handler(const Vship_Colorspace_t& src, const Vship_Colorspa
     VSHIP_HIP_CHECK(hipMemcpy(d_score_jobs_, &sj, sizeof(sj), hipMemcpyHostToDevice));
     VSHIP_HIP_CHECK(hipMemcpy(d_counts_, counts, sizeof(counts), hipMemcpyHostToDevice));
     VSHIP_HIP_CHECK(hipMemset(d_features_, 0, 108 * sizeof(double)));
+    FrontJob& j = *h_front_jobs_;
+    std::memset(&j, 0, sizeof(j));
     for (int i = 0; i < 2; i++) {
-        FrontJob& j = h_front_jobs_[i];
-        std::memset(&j, 0, sizeof(j));
-        j.out = xyb_[i];
-        j.width = width_;
-        j.height = height_;
-        j.tiles_x = layout_.front_tiles_x;
-        j.tile_start = i * layout_.front_tiles_per_image;
-        j.pair = 0;
-        j.color = i;
+        j.out[i] = xyb_[i];
+        j.color[i] = i;
     }
+    j.width = width_;
+    j.height = height_;
+    j.tiles_x = layout_.front_tiles_x;
+    j.tile_start = 0;
+    j.pair = 0;
 }
 
 Ssimu2Handler::~Ssimu2Handler()
@@ -309,7 +402,7 @@ void Ssimu2Handler::ensure_staging(int image, const int64_t ls[3])
 {
     const ImageSpec& s = spec_[image];
     for (int p = 0; p < 3; p++) {
-        const int64_t row = s.plane_w[p] * s.fc.bytes;
+        const int64_t row = s.plane_w[p] * s.bytes;
         if (ls[p] < row) VSHIP_THROW(Vship_BadPointer, "lineSize smaller than one row of samples");
         const size_t need = (size_t)(ls[p] * s.plane_h[p]);
         if (need > staging_cap_[image][p]) {
@@ -325,9 +418,9 @@ void Ssimu2Handler::ensure_staging(int image, const int64_t ls[3])
 
 void Ssimu2Handler::launch_all()
 {
-    VSHIP_HIP_CHECK(hipMemcpyAsync(d_front_jobs_, h_front_jobs_, 2 * sizeof(FrontJob), hipMemcpyHostToDevice, stream_));
-    VSHIP_HIP_CHECK(launch_frontend<float>(d_front_jobs_, d_counts_, 2 * layout_.front_tiles_per_image, d_colors_,
-                                           d_geom_, front_lds_, stream_));
+    VSHIP_HIP_CHECK(hipMemcpyAsync(d_front_jobs_, h_front_jobs_, sizeof(FrontJob), hipMemcpyHostToDevice, stream_));
+    VSHIP_HIP_CHECK(launch_frontend<float>(d_front_jobs_, d_counts_, layout_.front_tiles_per_image, d_colors_, d_geom_,
+                                           front_lds_, stream_));
     VSHIP_HIP_CHECK(launch_blur<float>(layout_.blur_config, d_maps_, d_counts_, blur_grid_for(layout_.blur_tiles),
                                        taps_, d_partials_, stream_));
     VSHIP_HIP_CHECK(launch_score(d_score_jobs_, d_counts_, 1, d_maps_, d_partials_, d_features_, d_score_, stream_));
@@ -337,10 +430,10 @@ void Ssimu2Handler::run(const uint8_t* const p1[3], const uint8_t* const p2[3], 
                         const int64_t ls2[3])
 {
     for (int p = 0; p < 3; p++) {
-        h_front_jobs_[0].plane[p] = p1[p];
-        h_front_jobs_[0].stride[p] = ls1[p];
-        h_front_jobs_[1].plane[p] = p2[p];
-        h_front_jobs_[1].stride[p] = ls2[p];
+        h_front_jobs_->plane[0][p] = p1[p];
+        h_front_jobs_->stride[0][p] = ls1[p];
+        h_front_jobs_->plane[1][p] = p2[p];
+        h_front_jobs_->stride[1][p] = ls2[p];
     }
     launch_all();
     VSHIP_HIP_CHECK(hipMemcpyAsync(h_score_, d_score_, sizeof(double), hipMemcpyDeviceToHost, stream_));
@@ -361,7 +454,7 @@ double Ssimu2Handler::compute_host(const uint8_t* const src1[3], const uint8_t* 
         for (int p = 0; p < 3; p++) {
             if (srcs[i][p] == nullptr) VSHIP_THROW(Vship_BadPointer, "null plane pointer");
             // same extent as the reference's upload (vshipColor.hpp:96)
-            const size_t bytes = (size_t)(lss[i][p] * s.plane_h[p] - (lss[i][p] - s.fc.bytes * s.plane_w[p]));
+            const size_t bytes = (size_t)(lss[i][p] * s.plane_h[p] - (lss[i][p] - s.bytes * s.plane_w[p]));
             VSHIP_HIP_CHECK(hipMemcpyAsync(staging_[i][p], srcs[i][p], bytes, hipMemcpyHostToDevice, stream_));
             dev[i][p] = staging_[i][p];
         }
@@ -376,7 +469,7 @@ double Ssimu2Handler::compute_device(const uint8_t* const src1[3], const uint8_t
     VSHIP_HIP_CHECK(hipSetDevice(device_));
     for (int p = 0; p < 3; p++) {
         if (src1[p] == nullptr || src2[p] == nullptr) VSHIP_THROW(Vship_BadPointer, "null plane pointer");
-        if (ls1[p] < spec_[0].plane_w[p] * spec_[0].fc.bytes || ls2[p] < spec_[1].plane_w[p] * spec_[1].fc.bytes)
+        if (ls1[p] < spec_[0].plane_w[p] * spec_[0].bytes || ls2[p] < spec_[1].plane_w[p] * spec_[1].bytes)
             VSHIP_THROW(Vship_BadPointer, "lineSize smaller than one row of samples");
     }
     run(src1, src2, ls1, ls2);
@@ -412,8 +505,8 @@ void Ssimu2Handler::profile(int iters, double* ms_front, double* ms_blur, double
     double acc[3] = {0, 0, 0};
     for (int it = 0; it < iters; it++) {
         VSHIP_HIP_CHECK(hipEventRecord(ev[0], stream_));
-        VSHIP_HIP_CHECK(launch_frontend<float>(d_front_jobs_, d_counts_, 2 * layout_.front_tiles_per_image,
-                                               d_colors_, d_geom_, front_lds_, stream_));
+        VSHIP_HIP_CHECK(launch_frontend<float>(d_front_jobs_, d_counts_, layout_.front_tiles_per_image, d_colors_,
+                                               d_geom_, front_lds_, stream_));
         VSHIP_HIP_CHECK(hipEventRecord(ev[1], stream_));
         VSHIP_HIP_CHECK(launch_blur<float>(layout_.blur_config, d_maps_, d_counts_, blur_grid_for(layout_.blur_tiles),
                                            taps_, d_partials_, stream_));
@@ -438,7 +531,7 @@ void Ssimu2Handler::work(double* bytes, double* flops) const
 {
     double in_bytes = 0.0;
     for (int i = 0; i < 2; i++)
-        for (int p = 0; p < 3; p++) in_bytes += (double)spec_[i].plane_w[p] * spec_[i].plane_h[p] * spec_[i].fc.bytes;
+        for (int p = 0; p < 3; p++) in_bytes += (double)spec_[i].plane_w[p] * spec_[i].plane_h[p] * spec_[i].bytes;
     double active = 0.0, fl = 0.0, blocks = 0.0;
     for (const auto& m : layout_.maps) {
         const double px = (double)m.w * m.h;

@@ -21,6 +21,7 @@ namespace vship {
 // Validated per-image colour description.
 struct ImageSpec {
     FrontColor fc;
+    int bytes;                    // bytes per sample
     int64_t width, height;        // input size (level 0 when no crop/resize)
     int64_t plane_w[3], plane_h[3];
 };
@@ -93,7 +94,7 @@ private:
     float* d_partials_ = nullptr;
     double* d_features_ = nullptr;
     double* d_score_ = nullptr;
-    FrontJob* h_front_jobs_ = nullptr;         // pinned
+    FrontJob* h_front_jobs_ = nullptr;         // pinned, one pair
     double* h_score_ = nullptr;                // pinned
     uint8_t* staging_[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
     size_t staging_cap_[2][3] = {{0, 0, 0}, {0, 0, 0}};

@@ -1,65 +1,59 @@
 // frontend.hip - fused colour front-end + 5-level pyramid + XYB for gfx950.
 //
-// One workgroup of 256 threads owns a 64x64 level-0 tile of one image; thread
-// (tx, ty) owns the 4x4 block at (4tx, 4ty). Per tile, in one pass:
+// One workgroup of 1024 threads owns a 64x64 level-0 tile of both images of a
+// pair; thread (tx, ty) owns the 2x2 block at (2tx, 2ty). Per tile and image:
 //   raw samples -> range (rangeToFull.hpp:24-55) -> chroma Catmull-Rom
-//   upsampling through LDS (chromaUpsample.hpp:28-247) -> matrix + EOTF
-//   (YUVToLinRGB.hpp:157-181) -> primaries (primaries.hpp:134-158)
-//   -> linear-RGB 2x2 box pyramid with the edge clamp (downsample.hpp:3-19):
-//      level 1 and 2 in registers, levels 3 and 4 through LDS
-//   -> positive XYB (makeXYB.hpp:6-64) of exactly the planes the score uses.
+//   upsampling through LDS (chromaUpsample.hpp:28-247) -> matrix, EOTF,
+//   linear-light matrix and primaries (YUVToLinRGB.hpp:157-181,
+//   primaries.hpp:134-158; the matrices are folded on the host into one 3x3
+//   before and one after the EOTF) -> linear-RGB 2x2 box pyramid with the
+//   edge clamp (downsample.hpp:3-19): level 1 in registers, 2 to 4 through
+//   LDS -> positive XYB (makeXYB.hpp:6-64) of exactly the planes the
+//   score uses.
 // A 64x64 level-0 tile maps onto whole 4x4 level-4 blocks, and the edge clamp
 // of every level only reads pixels inside the same tile, so tiles are
-// independent. The reference instead writes float planes of every level and
-// channel back to HBM between ~40 launches per pair.
+// independent. Both images' XYB values stay in registers until the tile is
+// stored, which the FP16 layout needs (distorted = fp16(b - a) of the FP32
+// values). Only the raw-sample loads are specialised per sample type and
+// the EOTF switch sits outside the 16-pixel loop, which keeps the kernel small
+// enough for the instruction cache.
 #include "ssimu2_common.hpp"
 #include "color_device.hpp"
 #include "kernels.hpp"
+
+#define VSHIP_INLINE __device__ __attribute__((always_inline)) inline
 
 namespace vship {
 
 namespace {
 
-struct ChromaStage {
-    int ccols, crows;   // staged chroma block incl. 2-sample halo each side
-    int cx0, cy0;       // chroma coordinates of staged element (0, 0)
-    bool mid;           // horizontally upsampled rows staged too
-};
+VSHIP_INLINE int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-__device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
-
-// Segment parameter of full-resolution coordinate X for decimation 2^sub
-// (sub >= 1), cosited or centred siting: X maps to segment [k, k+1] at t.
-// Cosited: the sample k sits on X = k * 2^sub (kernels *LeftUpscale*,
-// *TopUpscale*); centred: on X = k * 2^sub + 2^(sub-1) - 0.5
-// (kernels *CenterUpscale*), chromaUpsample.hpp:54-149.
-__device__ inline void seg_of(int X, int sub, bool cosited, int& k, float& t, bool& copy)
+// Segment of full-resolution coordinate X for decimation 2^sub (sub >= 1):
+// X maps to segment [k, k+1] at parameter t. Cosited: sample k sits on
+// X = k * 2^sub (kernels *LeftUpscale*, *TopUpscale*); centred: on
+// X = k * 2^sub + 2^(sub-1) - 0.5 (kernels *CenterUpscale*),
+// chromaUpsample.hpp:54-149.
+VSHIP_INLINE void seg_of(int X, int sub, bool cosited, int& k, float& t, bool& copy)
 {
     const int mask = (1 << sub) - 1;
+    const float inv = sub == 1 ? 0.5f : 0.25f;
     if (cosited) {
         k = X >> sub;
         const int r = X & mask;
         copy = (r == 0);
-        t = (float)r * (1.0f / (float)(1 << sub));
+        t = (float)r * inv;
     } else {
         const int off = 1 << (sub - 1);
         k = (X - off) >> sub;  // arithmetic shift: X < off gives k = -1
         const int r = (X - off) & mask;
         copy = false;
-        t = ((float)r + 0.5f) * (1.0f / (float)(1 << sub));
+        t = ((float)r + 0.5f) * inv;
     }
 }
 
-template <typename StoreT>
-__device__ inline void store_row4(StoreT* dst, const float v[4], int xg, int width)
-{
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-        if (xg + i < width) dst[i] = store_cast<StoreT>(v[i]);
-}
-
 template <int KIND>
-__device__ inline float load_kind(const uint8_t* p, uint32_t mask)
+VSHIP_INLINE float load_kind(const uint8_t* p, uint32_t mask)
 {
     if constexpr (KIND == kSampleU8) {
         return (float)p[0];
@@ -68,304 +62,379 @@ __device__ inline float load_kind(const uint8_t* p, uint32_t mask)
     } else if constexpr (KIND == kSampleHalf) {
         return __half2float(__ushort_as_half((uint16_t)((uint32_t)p[0] | ((uint32_t)p[1] << 8))));
     } else {
-        return __uint_as_float((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24));
+        return __uint_as_float((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) |
+                               ((uint32_t)p[3] << 24));
     }
 }
 
-template <typename StoreT, int KIND>
-__device__ __attribute__((always_inline)) void front_tile(const FrontJob& job, const FrontColor& fc, const PairGeom& geo, int tile_x, int tile_y,
-                           float* lds)
+template <int KIND>
+constexpr int kind_bytes() { return KIND == kSampleU8 ? 1 : (KIND == kSampleFloat ? 4 : 2); }
+
+// 2x2 block of one plane at full resolution (clamped at the image edge).
+template <int KIND>
+VSHIP_INLINE void load_block(const uint8_t* base, int64_t stride, int x0, int y0, int W, int H, uint32_t mask,
+                             const RangeXform& rx, float out[2][2])
 {
-    constexpr int kBytes = (KIND == kSampleU8) ? 1 : (KIND == kSampleFloat ? 4 : 2);
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const uint8_t* row = base + (int64_t)min(y0 + j, H - 1) * stride;
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+            out[j][i] = apply_range(load_kind<KIND>(row + min(x0 + i, W - 1) * kind_bytes<KIND>(), mask), rx);
+    }
+}
+
+// Both chroma planes of the tile (with halo) into LDS, clamped coordinates.
+template <int KIND>
+VSHIP_INLINE void stage_chroma(const uint8_t* const* plane, const int64_t* pstride, uint32_t mask,
+                               const RangeXform& rx, float* s_chroma, int ccols, int crows, int cx0, int cy0, int cw,
+                               int ch)
+{
     const int t = threadIdx.x;
-    const int tx = t & 15, ty = t >> 4;
-    const int W = job.width, H = job.height;
-    const int x0 = tile_x * kFrontTile + 4 * tx;
-    const int y0 = tile_y * kFrontTile + 4 * ty;
-    const uint32_t mask = (fc.bits >= 16) ? 0xffffu : ((1u << fc.bits) - 1u);
-
-        float P[3][4][4];  // plane values, then linear R, G, B of the 4x4 block
-
-        // plane 0 (and all planes without subsampling) straight from HBM
-        const bool sub = fc.subw || fc.subh;
+    const int tx = t & 31, ty = t >> 5;
 #pragma unroll
-        for (int p = 0; p < 3; p++) {
-            if (p > 0 && sub) break;
-            const RangeXform rx = make_range(fc.range, fc.bits, p != 0 && fc.yuv);
-            const uint8_t* base = job.plane[p];
-            const int64_t stride = job.stride[p];
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const uint8_t* row = base + (int64_t)min(y0 + j, H - 1) * stride;
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-                    P[p][j][i] = apply_range(load_kind<KIND>(row + min(x0 + i, W - 1) * kBytes, mask), rx);
-            }
+    for (int p = 0; p < 2; p++) {
+        const uint8_t* base = plane[1 + p];
+        const int64_t stride = pstride[1 + p];
+        float* dst = s_chroma + p * ccols * crows;
+        for (int r = ty; r < crows; r += 32) {
+            const uint8_t* row = base + (int64_t)clampi(cy0 + r, 0, ch - 1) * stride;
+            for (int c = tx; c < ccols; c += 32)
+                dst[r * ccols + c] =
+                    apply_range(load_kind<KIND>(row + clampi(cx0 + c, 0, cw - 1) * kind_bytes<KIND>(), mask), rx);
         }
+    }
+}
 
-        if (sub) {
-            // stage both chroma planes (decoded, range-normalised) with a
-            // 2-sample halo; clamped coordinates reproduce the reference's
-            // edge rules (chromaUpsample.hpp:29-35,41-47)
-            ChromaStage cs;
-            cs.ccols = (kFrontTile >> fc.subw) + 4;
-            cs.crows = (kFrontTile >> fc.subh) + 4;
-            cs.cx0 = ((tile_x * kFrontTile) >> fc.subw) - 2;
-            cs.cy0 = ((tile_y * kFrontTile) >> fc.subh) - 2;
-            cs.mid = fc.subw && fc.subh;
-            const int cw = ((W - 1) >> fc.subw) + 1;
-            const int ch = ((H - 1) >> fc.subh) + 1;
-            const int plane_elems = cs.ccols * cs.crows;
-            float* s_chroma = lds;
-            float* s_mid = lds + 2 * plane_elems;
-            const RangeXform rx = make_range(fc.range, fc.bits, fc.yuv != 0);
-            __syncthreads();  // previous item may still read LDS
+// EOTF of the whole 2x2x3 block, one code path per transfer.
+template <int TRC>
+VSHIP_INLINE void eotf_block(float P[3][2][2])
+{
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int i = 0; i < 2; i++) P[c][j][i] = eotf(P[c][j][i], TRC);
+}
+
+VSHIP_INLINE void apply_transfer(int trc, float P[3][2][2])
+{
+    switch (trc) {
+    case 1: eotf_block<1>(P); break;
+    case 4: eotf_block<4>(P); break;
+    case 5: case 6: eotf_block<5>(P); break;
+    case 7: eotf_block<7>(P); break;
+    case 13: eotf_block<13>(P); break;
+    case 16: eotf_block<16>(P); break;
+    case 17: eotf_block<17>(P); break;
+    case 18:
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int i = 0; i < 2; i++) eotf_hlg(P[0][j][i], P[1][j][i], P[2][j][i]);
+        break;
+    default: break;  // linear
+    }
+}
+
+// XYB values of one image's tile in the thread's ownership (stored planes only).
+struct TileXyb {
+    float y0[2][2];   // level 0: Y of the 2x2 block
+    float x1, y1;     // level 1: X, Y of one pixel
+    float x2, y2;     // level 2 (threads < 256)
+    float l3[3];      // level 3 (threads < 64): X, Y, B
+    float l4[3];      // level 4 (threads < 16): X, Y, B
+};
+
+// Level s+1 pixel from the 2x2 parents at (2lx, 2ly) of a level stored in LDS
+// as [3][n][n + 1], with the edge clamp min(2x+1, w-1) (downsample.hpp:3-19).
+VSHIP_INLINE void down_lds(const float* src, int n, int lx, int ly, int gx2, int gy2, int w, int h, float out[3])
+{
+    const int xb = (gx2 + 1 < w) ? 2 * lx + 1 : 2 * lx;
+    const int yb = (gy2 + 1 < h) ? 2 * ly + 1 : 2 * ly;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float* b = src + c * n * (n + 1);
+        float s = b[(2 * ly) * (n + 1) + 2 * lx];
+        s += b[yb * (n + 1) + 2 * lx];
+        s += b[(2 * ly) * (n + 1) + xb];
+        s += b[yb * (n + 1) + xb];
+        out[c] = s * 0.25f;
+    }
+}
+
+VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc, const PairGeom& geo, int tile_x,
+                               int tile_y, float* lds, TileXyb& res)
+{
+    const uint8_t* const* plane = job.plane[k];
+    const int64_t* pstride = job.stride[k];
+    const int t = threadIdx.x;
+    const int tx = t & 31, ty = t >> 5;
+    const int W = job.width, H = job.height;
+    const int x0 = tile_x * kFrontTile + 2 * tx;
+    const int y0 = tile_y * kFrontTile + 2 * ty;
+    const RangeXform r0 = {fc.range0[0], fc.range0[1], fc.range0[2]};
+    const RangeXform r12 = {fc.range12[0], fc.range12[1], fc.range12[2]};
+
+    float P[3][2][2];  // plane values, then linear R, G, B of the 2x2 block
+    const bool sub = fc.subw || fc.subh;
+    switch (fc.sample) {
+    case kSampleU8:
+        load_block<kSampleU8>(plane[0], pstride[0], x0, y0, W, H, fc.mask, r0, P[0]);
+        if (!sub) {
+            load_block<kSampleU8>(plane[1], pstride[1], x0, y0, W, H, fc.mask, r12, P[1]);
+            load_block<kSampleU8>(plane[2], pstride[2], x0, y0, W, H, fc.mask, r12, P[2]);
+        }
+        break;
+    case kSampleU16:
+        load_block<kSampleU16>(plane[0], pstride[0], x0, y0, W, H, fc.mask, r0, P[0]);
+        if (!sub) {
+            load_block<kSampleU16>(plane[1], pstride[1], x0, y0, W, H, fc.mask, r12, P[1]);
+            load_block<kSampleU16>(plane[2], pstride[2], x0, y0, W, H, fc.mask, r12, P[2]);
+        }
+        break;
+    case kSampleHalf:
+        load_block<kSampleHalf>(plane[0], pstride[0], x0, y0, W, H, fc.mask, r0, P[0]);
+        if (!sub) {
+            load_block<kSampleHalf>(plane[1], pstride[1], x0, y0, W, H, fc.mask, r12, P[1]);
+            load_block<kSampleHalf>(plane[2], pstride[2], x0, y0, W, H, fc.mask, r12, P[2]);
+        }
+        break;
+    default:
+        load_block<kSampleFloat>(plane[0], pstride[0], x0, y0, W, H, fc.mask, r0, P[0]);
+        if (!sub) {
+            load_block<kSampleFloat>(plane[1], pstride[1], x0, y0, W, H, fc.mask, r12, P[1]);
+            load_block<kSampleFloat>(plane[2], pstride[2], x0, y0, W, H, fc.mask, r12, P[2]);
+        }
+        break;
+    }
+
+    if (sub) {
+        // both chroma planes staged in LDS with a 2-sample halo; clamped
+        // coordinates reproduce the reference's edge rules
+        // (chromaUpsample.hpp:29-35,41-47)
+        const int ccols = (kFrontTile >> fc.subw) + 4;
+        const int crows = (kFrontTile >> fc.subh) + 4;
+        const int cx0 = ((tile_x * kFrontTile) >> fc.subw) - 2;
+        const int cy0 = ((tile_y * kFrontTile) >> fc.subh) - 2;
+        const int cw = ((W - 1) >> fc.subw) + 1;
+        const int ch = ((H - 1) >> fc.subh) + 1;
+        const int plane_elems = ccols * crows;
+        float* s_chroma = lds;
+        float* s_mid = lds + 2 * plane_elems;
+        switch (fc.sample) {
+        case kSampleU8: stage_chroma<kSampleU8>(plane, pstride, fc.mask, r12, s_chroma, ccols, crows, cx0, cy0, cw, ch); break;
+        case kSampleU16: stage_chroma<kSampleU16>(plane, pstride, fc.mask, r12, s_chroma, ccols, crows, cx0, cy0, cw, ch); break;
+        case kSampleHalf: stage_chroma<kSampleHalf>(plane, pstride, fc.mask, r12, s_chroma, ccols, crows, cx0, cy0, cw, ch); break;
+        default: stage_chroma<kSampleFloat>(plane, pstride, fc.mask, r12, s_chroma, ccols, crows, cx0, cy0, cw, ch); break;
+        }
+        __syncthreads();
+        const bool both = fc.subw && fc.subh;
+        if (both) {
+            // horizontal pass over every staged chroma row first, like the
+            // reference (chromaUpsample.hpp:165-202)
+            const int X = t & (kFrontTile - 1);
+            int kk; float tt; bool copy;
+            seg_of(tile_x * kFrontTile + X, fc.subw, fc.h_cosited != 0, kk, tt, copy);
+            const int c = kk - cx0;
 #pragma unroll
             for (int p = 0; p < 2; p++) {
-                const uint8_t* base = job.plane[1 + p];
-                const int64_t stride = job.stride[1 + p];
-                for (int r = ty; r < cs.crows; r += 16) {
-                    const uint8_t* row = base + (int64_t)clampi(cs.cy0 + r, 0, ch - 1) * stride;
-                    for (int c = tx; c < cs.ccols; c += 16)
-                        s_chroma[p * plane_elems + r * cs.ccols + c] =
-                            apply_range(load_kind<KIND>(row + clampi(cs.cx0 + c, 0, cw - 1) * kBytes, mask), rx);
+                for (int r = t / kFrontTile; r < crows; r += kFrontThreads / kFrontTile) {
+                    const float* row = s_chroma + p * plane_elems + r * ccols;
+                    s_mid[(p * crows + r) * kFrontTile + X] =
+                        copy ? row[c] : catmull_rom(row[c - 1], row[c], row[c + 1], row[c + 2], tt);
                 }
             }
             __syncthreads();
-            if (cs.mid) {
-                // horizontal pass for every staged chroma row (reference runs
-                // it first, into a temporary plane, chromaUpsample.hpp:165-202)
-                const int mid_elems = cs.crows * kFrontTile;
-                const int X = t & (kFrontTile - 1);
-                int k; float tt; bool copy;
-                seg_of(tile_x * kFrontTile + X, fc.subw, fc.h_cosited != 0, k, tt, copy);
-                const int c = k - cs.cx0;
-#pragma unroll
-                for (int p = 0; p < 2; p++) {
-                    for (int r = t >> 6; r < cs.crows; r += kFrontThreads / kFrontTile) {
-                        const float* row = s_chroma + p * plane_elems + r * cs.ccols;
-                        s_mid[p * mid_elems + r * kFrontTile + X] =
-                            copy ? row[c] : catmull_rom(row[c - 1], row[c], row[c + 1], row[c + 2], tt);
-                    }
-                }
-                __syncthreads();
-            }
-            // final chroma values of the thread's 4x4 block
-#pragma unroll
-            for (int p = 0; p < 2; p++) {
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int Y = y0 + j;
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const int X = x0 + i;
-                        float val;
-                        if (fc.subh == 0) {
-                            const int r = Y - cs.cy0;
-                            const float* row = s_chroma + p * plane_elems + r * cs.ccols;
-                            int k; float tt; bool copy;
-                            seg_of(X, fc.subw, fc.h_cosited != 0, k, tt, copy);
-                            const int c = k - cs.cx0;
-                            val = copy ? row[c] : catmull_rom(row[c - 1], row[c], row[c + 1], row[c + 2], tt);
-                        } else {
-                            int k; float tt; bool copy;
-                            seg_of(Y, fc.subh, fc.v_cosited != 0, k, tt, copy);
-                            const int r = k - cs.cy0;
-                            const float* colbase;
-                            int pitch;
-                            if (cs.mid) {
-                                colbase = s_mid + p * cs.crows * kFrontTile + (X - tile_x * kFrontTile);
-                                pitch = kFrontTile;
-                            } else {
-                                colbase = s_chroma + p * plane_elems + (X - cs.cx0);
-                                pitch = cs.ccols;
-                            }
-                            val = copy ? colbase[r * pitch]
-                                       : catmull_rom(colbase[(r - 1) * pitch], colbase[r * pitch],
-                                                     colbase[(r + 1) * pitch], colbase[(r + 2) * pitch], tt);
-                        }
-                        P[1 + p][j][i] = val;
-                    }
-                }
-            }
         }
-
-        // matrix + EOTF + primaries, in place (P becomes linear R, G, B)
+        // segment parameters of the thread's 2 columns and 2 rows
+        int kx[2], ky[2];
+        float txp[2], typ[2];
+        bool cpx[2], cpy[2];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                float R, G, B;
-                to_linear_rgb(P[0][j][i], P[1][j][i], P[2][j][i], fc.matrix, fc.transfer, R, G, B);
-                if (fc.use_prim) {
-                    const float r2 = fmaf(R, fc.prim[0], fmaf(G, fc.prim[1], B * fc.prim[2]));
-                    const float g2 = fmaf(R, fc.prim[3], fmaf(G, fc.prim[4], B * fc.prim[5]));
-                    const float b2 = fmaf(R, fc.prim[6], fmaf(G, fc.prim[7], B * fc.prim[8]));
-                    R = r2; G = g2; B = b2;
-                }
-                P[0][j][i] = R; P[1][j][i] = G; P[2][j][i] = B;
-            }
+        for (int i = 0; i < 2; i++) {
+            if (fc.subw) seg_of(x0 + i, fc.subw, fc.h_cosited != 0, kx[i], txp[i], cpx[i]);
+            else { kx[i] = x0 + i; txp[i] = 0.0f; cpx[i] = true; }
+            if (fc.subh) seg_of(y0 + i, fc.subh, fc.v_cosited != 0, ky[i], typ[i], cpy[i]);
+            else { ky[i] = y0 + i; typ[i] = 0.0f; cpy[i] = true; }
         }
-
-        StoreT* out = static_cast<StoreT*>(job.out);
-
-        // ---- level 0
-        {
-            const bool nx = geo.off[0][0] >= 0, ny = geo.off[0][1] >= 0, nb = geo.off[0][2] >= 0;
-            if (nx || ny || nb) {
-                float vx[4][4], vy[4][4], vb[4][4];
 #pragma unroll
-                for (int j = 0; j < 4; j++)
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const XYB q = to_xyb(P[0][j][i], P[1][j][i], P[2][j][i], nb);
-                        vx[j][i] = q.x; vy[j][i] = q.y; vb[j][i] = q.b;
-                    }
-                const int pitch = geo.pitch[0];
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    if (y0 + j >= H) break;
-                    const int64_t ro = (int64_t)(y0 + j) * pitch + x0;
-                    if (nx) store_row4(out + geo.off[0][0] + ro, vx[j], x0, W);
-                    if (ny) store_row4(out + geo.off[0][1] + ro, vy[j], x0, W);
-                    if (nb) store_row4(out + geo.off[0][2] + ro, vb[j], x0, W);
-                }
-            }
-        }
-
-        // ---- level 1 (2x2 per thread), edge clamp min(2x+1, w-1)
-        float L1[3][2][2];
-        {
-            const int w0 = W, h0 = H;
+        for (int p = 0; p < 2; p++) {
 #pragma unroll
             for (int j = 0; j < 2; j++) {
-                const bool ylast = !(y0 + 2 * j + 1 < h0);
 #pragma unroll
                 for (int i = 0; i < 2; i++) {
-                    const bool xlast = !(x0 + 2 * i + 1 < w0);
-#pragma unroll
-                    for (int c = 0; c < 3; c++) {
-                        const float p00 = P[c][2 * j][2 * i];
-                        const float p10 = ylast ? p00 : P[c][2 * j + 1][2 * i];
-                        const float p01 = xlast ? p00 : P[c][2 * j][2 * i + 1];
-                        const float p11 = ylast ? p01 : (xlast ? p10 : P[c][2 * j + 1][2 * i + 1]);
-                        float s = p00;
-                        s += p10;
-                        s += p01;
-                        s += p11;
-                        L1[c][j][i] = s * 0.25f;
+                    float val;
+                    if (fc.subh == 0) {
+                        const float* row = s_chroma + p * plane_elems + (y0 + j - cy0) * ccols;
+                        const int c = kx[i] - cx0;
+                        val = cpx[i] ? row[c] : catmull_rom(row[c - 1], row[c], row[c + 1], row[c + 2], txp[i]);
+                    } else {
+                        const int r = ky[j] - cy0;
+                        const float* colbase;
+                        int pitch;
+                        if (both) {
+                            colbase = s_mid + p * crows * kFrontTile + (x0 + i - tile_x * kFrontTile);
+                            pitch = kFrontTile;
+                        } else {
+                            colbase = s_chroma + p * plane_elems + (x0 + i - cx0);
+                            pitch = ccols;
+                        }
+                        val = cpy[j] ? colbase[r * pitch]
+                                     : catmull_rom(colbase[(r - 1) * pitch], colbase[r * pitch],
+                                                   colbase[(r + 1) * pitch], colbase[(r + 2) * pitch], typ[j]);
                     }
-                }
-            }
-            const int w1 = geo.w[1], h1 = geo.h[1];
-            const int x1 = x0 >> 1, y1 = y0 >> 1;
-            const bool nx = geo.off[1][0] >= 0, ny = geo.off[1][1] >= 0, nb = geo.off[1][2] >= 0;
-            if (nx || ny || nb) {
-#pragma unroll
-                for (int j = 0; j < 2; j++) {
-                    if (y1 + j >= h1) break;
-#pragma unroll
-                    for (int i = 0; i < 2; i++) {
-                        if (x1 + i >= w1) break;
-                        const XYB q = to_xyb(L1[0][j][i], L1[1][j][i], L1[2][j][i], nb);
-                        const int64_t o = (int64_t)(y1 + j) * geo.pitch[1] + x1 + i;
-                        if (nx) out[geo.off[1][0] + o] = store_cast<StoreT>(q.x);
-                        if (ny) out[geo.off[1][1] + o] = store_cast<StoreT>(q.y);
-                        if (nb) out[geo.off[1][2] + o] = store_cast<StoreT>(q.b);
-                    }
+                    P[1 + p][j][i] = val;
                 }
             }
         }
+    }
 
-        // ---- level 2 (1 per thread)
+    // matrix, EOTF, linear-light matrix with primaries (P becomes linear RGB)
+    if (fc.use_pre) {
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int i = 0; i < 2; i++) mat3_apply(fc.pre, P[0][j][i], P[1][j][i], P[2][j][i]);
+    }
+    apply_transfer(fc.transfer, P);
+    if (fc.use_post) {
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int i = 0; i < 2; i++) mat3_apply(fc.post, P[0][j][i], P[1][j][i], P[2][j][i]);
+    }
+
+    // ---- level 0: Y only
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int i = 0; i < 2; i++) res.y0[j][i] = to_xyb(P[0][j][i], P[1][j][i], P[2][j][i], false).y;
+
+    // ---- level 1 (1 per thread), edge clamp min(2x+1, w-1)
+    float L1[3];
+    {
+        const bool ylast = !(y0 + 1 < H);
+        const bool xlast = !(x0 + 1 < W);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float p00 = P[c][0][0];
+            const float p10 = ylast ? p00 : P[c][1][0];
+            const float p01 = xlast ? p00 : P[c][0][1];
+            const float p11 = ylast ? p01 : (xlast ? p10 : P[c][1][1]);
+            float s = p00;
+            s += p10;
+            s += p01;
+            s += p11;
+            L1[c] = s * 0.25f;
+        }
+        const XYB q = to_xyb(L1[0], L1[1], L1[2], false);
+        res.x1 = q.x;
+        res.y1 = q.y;
+    }
+
+    // ---- levels 2, 3, 4 through LDS (32x32 -> 16x16 -> 8x8 -> 4x4 per tile)
+    float* s_l1 = lds;                       // [3][32][33]
+    float* s_l2 = s_l1 + 3 * 32 * 33;        // [3][16][17]
+    float* s_l3 = s_l2 + 3 * 16 * 17;        // [3][8][9]
+    __syncthreads();                         // chroma staging no longer needed
+#pragma unroll
+    for (int c = 0; c < 3; c++) s_l1[c * 32 * 33 + ty * 33 + tx] = L1[c];
+    __syncthreads();
+    if (t < 256) {
+        const int lx = t & 15, ly = t >> 4;
         float L2[3];
-        {
-            const int w1 = geo.w[1], h1 = geo.h[1];
-            const int x1 = x0 >> 1, y1 = y0 >> 1;
-            const bool xlast = !(x1 + 1 < w1);
-            const bool ylast = !(y1 + 1 < h1);
+        down_lds(s_l1, 32, lx, ly, tile_x * 32 + 2 * lx, tile_y * 32 + 2 * ly, geo.w[1], geo.h[1], L2);
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const float p00 = L1[c][0][0];
-                const float p10 = ylast ? p00 : L1[c][1][0];
-                const float p01 = xlast ? p00 : L1[c][0][1];
-                const float p11 = ylast ? p01 : (xlast ? p10 : L1[c][1][1]);
-                float s = p00;
-                s += p10;
-                s += p01;
-                s += p11;
-                L2[c] = s * 0.25f;
-            }
-            const int x2 = x0 >> 2, y2 = y0 >> 2;
-            const bool nx = geo.off[2][0] >= 0, ny = geo.off[2][1] >= 0, nb = geo.off[2][2] >= 0;
-            if ((nx || ny || nb) && x2 < geo.w[2] && y2 < geo.h[2]) {
-                const XYB q = to_xyb(L2[0], L2[1], L2[2], nb);
-                const int64_t o = (int64_t)y2 * geo.pitch[2] + x2;
-                if (nx) out[geo.off[2][0] + o] = store_cast<StoreT>(q.x);
-                if (ny) out[geo.off[2][1] + o] = store_cast<StoreT>(q.y);
-                if (nb) out[geo.off[2][2] + o] = store_cast<StoreT>(q.b);
-            }
-        }
+        for (int c = 0; c < 3; c++) s_l2[c * 16 * 17 + ly * 17 + lx] = L2[c];
+        const XYB q = to_xyb(L2[0], L2[1], L2[2], false);
+        res.x2 = q.x;
+        res.y2 = q.y;
+    }
+    __syncthreads();
+    if (t < 64) {
+        const int lx = t & 7, ly = t >> 3;
+        float L3[3];
+        down_lds(s_l2, 16, lx, ly, tile_x * 16 + 2 * lx, tile_y * 16 + 2 * ly, geo.w[2], geo.h[2], L3);
+#pragma unroll
+        for (int c = 0; c < 3; c++) s_l3[c * 8 * 9 + ly * 9 + lx] = L3[c];
+        const XYB q = to_xyb(L3[0], L3[1], L3[2], true);
+        res.l3[0] = q.x; res.l3[1] = q.y; res.l3[2] = q.b;
+    }
+    __syncthreads();
+    if (t < 16) {
+        const int lx = t & 3, ly = t >> 2;
+        float L4[3];
+        down_lds(s_l3, 8, lx, ly, tile_x * 8 + 2 * lx, tile_y * 8 + 2 * ly, geo.w[3], geo.h[3], L4);
+        const XYB q = to_xyb(L4[0], L4[1], L4[2], true);
+        res.l4[0] = q.x; res.l4[1] = q.y; res.l4[2] = q.b;
+    }
+    __syncthreads();  // LDS is reused by the next image
+}
 
-        // ---- levels 3 and 4 through LDS (16x16 -> 8x8 -> 4x4 per tile)
-        float* s_l2 = lds;                 // [3][16][17]
-        float* s_l3 = lds + 3 * 16 * 17;   // [3][8][9]
-        __syncthreads();                   // chroma staging no longer needed
+// Stored value of a plane element: FP32 planes hold the values themselves;
+// FP16 planes hold fp16(a - offset) for the reference and fp16(b - a) for the
+// distorted image.
+template <typename StoreT>
+VSHIP_INLINE void put(StoreT* out0, StoreT* out1, int64_t o, int plane, float a, float b)
+{
+    if constexpr (sizeof(StoreT) == 4) {
+        out0[o] = store_cast<StoreT>(a);
+        out1[o] = store_cast<StoreT>(b);
+    } else {
+        out0[o] = store_cast<StoreT>(a - kHalfOffset[plane]);
+        out1[o] = store_cast<StoreT>(b - a);
+    }
+}
+
+template <typename StoreT>
+VSHIP_INLINE void store_pair(const FrontJob& job, const PairGeom& geo, int tile_x, int tile_y, const TileXyb& a,
+                             const TileXyb& b)
+{
+    const int t = threadIdx.x;
+    const int tx = t & 31, ty = t >> 5;
+    const int W = job.width, H = job.height;
+    const int x0 = tile_x * kFrontTile + 2 * tx;
+    const int y0 = tile_y * kFrontTile + 2 * ty;
+    StoreT* o0 = static_cast<StoreT*>(job.out[0]);
+    StoreT* o1 = static_cast<StoreT*>(job.out[1]);
 #pragma unroll
-        for (int c = 0; c < 3; c++) s_l2[c * 16 * 17 + ty * 17 + tx] = L2[c];
-        __syncthreads();
-        if (t < 64) {
-            const int lx = t & 7, ly = t >> 3;
-            const int x3 = tile_x * 8 + lx, y3 = tile_y * 8 + ly;
-            const int x2a = tile_x * 16 + 2 * lx, y2a = tile_y * 16 + 2 * ly;
-            const int xb = (x2a + 1 < geo.w[2]) ? 2 * lx + 1 : 2 * lx;
-            const int yb = (y2a + 1 < geo.h[2]) ? 2 * ly + 1 : 2 * ly;
-            float L3[3];
+    for (int j = 0; j < 2; j++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const float* b2 = s_l2 + c * 16 * 17;
-                float s = b2[(2 * ly) * 17 + 2 * lx];
-                s += b2[yb * 17 + 2 * lx];
-                s += b2[(2 * ly) * 17 + xb];
-                s += b2[yb * 17 + xb];
-                L3[c] = s * 0.25f;
-                s_l3[c * 8 * 9 + ly * 9 + lx] = L3[c];
-            }
-            const bool nx = geo.off[3][0] >= 0, ny = geo.off[3][1] >= 0, nb = geo.off[3][2] >= 0;
-            if ((nx || ny || nb) && x3 < geo.w[3] && y3 < geo.h[3]) {
-                const XYB q = to_xyb(L3[0], L3[1], L3[2], nb);
-                const int64_t o = (int64_t)y3 * geo.pitch[3] + x3;
-                if (nx) out[geo.off[3][0] + o] = store_cast<StoreT>(q.x);
-                if (ny) out[geo.off[3][1] + o] = store_cast<StoreT>(q.y);
-                if (nb) out[geo.off[3][2] + o] = store_cast<StoreT>(q.b);
-            }
+        for (int i = 0; i < 2; i++)
+            if (y0 + j < H && x0 + i < W)
+                put(o0, o1, geo.off[0][1] + (int64_t)(y0 + j) * geo.pitch[0] + x0 + i, 1, a.y0[j][i], b.y0[j][i]);
+    const int x1 = tile_x * 32 + tx, y1 = tile_y * 32 + ty;
+    if (y1 < geo.h[1] && x1 < geo.w[1]) {
+        const int64_t o = (int64_t)y1 * geo.pitch[1] + x1;
+        put(o0, o1, geo.off[1][0] + o, 0, a.x1, b.x1);
+        put(o0, o1, geo.off[1][1] + o, 1, a.y1, b.y1);
+    }
+    if (t < 256) {
+        const int x2 = tile_x * 16 + (t & 15), y2 = tile_y * 16 + (t >> 4);
+        if (x2 < geo.w[2] && y2 < geo.h[2]) {
+            const int64_t o = (int64_t)y2 * geo.pitch[2] + x2;
+            put(o0, o1, geo.off[2][0] + o, 0, a.x2, b.x2);
+            put(o0, o1, geo.off[2][1] + o, 1, a.y2, b.y2);
         }
-        __syncthreads();
-        if (t < 16) {
-            const int lx = t & 3, ly = t >> 2;
-            const int x4 = tile_x * 4 + lx, y4 = tile_y * 4 + ly;
-            const int x3a = tile_x * 8 + 2 * lx, y3a = tile_y * 8 + 2 * ly;
-            const int xb = (x3a + 1 < geo.w[3]) ? 2 * lx + 1 : 2 * lx;
-            const int yb = (y3a + 1 < geo.h[3]) ? 2 * ly + 1 : 2 * ly;
-            float L4[3];
+    }
+    if (t < 64) {
+        const int x3 = tile_x * 8 + (t & 7), y3 = tile_y * 8 + (t >> 3);
+        if (x3 < geo.w[3] && y3 < geo.h[3]) {
+            const int64_t o = (int64_t)y3 * geo.pitch[3] + x3;
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const float* b3 = s_l3 + c * 8 * 9;
-                float s = b3[(2 * ly) * 9 + 2 * lx];
-                s += b3[yb * 9 + 2 * lx];
-                s += b3[(2 * ly) * 9 + xb];
-                s += b3[yb * 9 + xb];
-                L4[c] = s * 0.25f;
-            }
-            const bool nx = geo.off[4][0] >= 0, ny = geo.off[4][1] >= 0, nb = geo.off[4][2] >= 0;
-            if ((nx || ny || nb) && x4 < geo.w[4] && y4 < geo.h[4]) {
-                const XYB q = to_xyb(L4[0], L4[1], L4[2], nb);
-                const int64_t o = (int64_t)y4 * geo.pitch[4] + x4;
-                if (nx) out[geo.off[4][0] + o] = store_cast<StoreT>(q.x);
-                if (ny) out[geo.off[4][1] + o] = store_cast<StoreT>(q.y);
-                if (nb) out[geo.off[4][2] + o] = store_cast<StoreT>(q.b);
-            }
+            for (int c = 0; c < 3; c++) put(o0, o1, geo.off[3][c] + o, c, a.l3[c], b.l3[c]);
         }
+    }
+    if (t < 16) {
+        const int x4 = tile_x * 4 + (t & 3), y4 = tile_y * 4 + (t >> 2);
+        if (x4 < geo.w[4] && y4 < geo.h[4]) {
+            const int64_t o = (int64_t)y4 * geo.pitch[4] + x4;
+#pragma unroll
+            for (int c = 0; c < 3; c++) put(o0, o1, geo.off[4][c] + o, c, a.l4[c], b.l4[c]);
+        }
+    }
 }
 
 template <typename StoreT>
@@ -384,18 +453,14 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
             if (jobs[mid].tile_start <= item) lo = mid; else hi = mid - 1;
         }
         const FrontJob& job = jobs[lo];
-        const FrontColor fc = colors[job.color];
         const PairGeom& geo = geoms[job.pair];
         const int local = item - job.tile_start;
-        const int tile_x = local % job.tiles_x;
-        const int tile_y = local / job.tiles_x;
-        switch (fc.sample) {
-        case kSampleU8: front_tile<StoreT, kSampleU8>(job, fc, geo, tile_x, tile_y, lds); break;
-        case kSampleU16: front_tile<StoreT, kSampleU16>(job, fc, geo, tile_x, tile_y, lds); break;
-        case kSampleHalf: front_tile<StoreT, kSampleHalf>(job, fc, geo, tile_x, tile_y, lds); break;
-        default: front_tile<StoreT, kSampleFloat>(job, fc, geo, tile_x, tile_y, lds); break;
-        }
-        __syncthreads();
+        const int tile_x = local % job.tiles_x, tile_y = local / job.tiles_x;
+        TileXyb a, b;
+        compute_tile(job, 0, colors[job.color[0]], geo, tile_x, tile_y, lds, a);
+        compute_tile(job, 1, colors[job.color[1]], geo, tile_x, tile_y, lds, b);
+        store_pair<StoreT>(job, geo, tile_x, tile_y, a, b);
+        __syncthreads();  // LDS is reused by the next item
     }
 }
 
@@ -409,7 +474,7 @@ size_t frontend_lds_bytes(const FrontColor& fc)
         chroma = 2 * ccols * crows;
         if (fc.subw && fc.subh) chroma += 2 * crows * kFrontTile;
     }
-    const size_t levels = 3 * 16 * 17 + 3 * 8 * 9;
+    const size_t levels = 3 * 32 * 33 + 3 * 16 * 17 + 3 * 8 * 9;
     return sizeof(float) * (chroma > levels ? chroma : levels);
 }
 

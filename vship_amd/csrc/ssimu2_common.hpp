@@ -20,9 +20,9 @@ constexpr int kUsedScales = 5;    // scale 5 carries no active map
 constexpr int kMaxJobsPerPair = 11;
 constexpr float kC2 = 0.0009f;    // SSIM constant (score.hpp:272-273)
 
-// Front-end tile: 64x64 level-0 pixels, 256 threads, each owning a 4x4 block.
+// Front-end tile: 64x64 level-0 pixels, 1024 threads, each owning a 2x2 block.
 constexpr int kFrontTile = 64;
-constexpr int kFrontThreads = 256;
+constexpr int kFrontThreads = 1024;
 
 // Blur/map tiles: 256 threads, output tile shape chosen per launch (blur.hip),
 // halo 8 on every side.
@@ -50,21 +50,27 @@ enum MapFlags : int32_t {
     kMapDetail = 4,
 };
 
-// Colour front-end description of one image (the device-side restatement of
-// Vship_Colorspace_t, reference src/VshipColor.h:81-96, after validation).
+// Colour front-end description of one image: the device-side form of
+// Vship_Colorspace_t (reference src/VshipColor.h:81-96) after validation. The
+// reference's colour steps are folded on the host into
+//   v' = (raw * scale - offset) * gain          per plane (range)
+//   e  = pre * v'                               Y'CbCr -> transfer inputs
+//   l  = EOTF(e)                                per channel (HLG: per triple)
+//   rgb = post * l                              linear-light matrix + primaries
+// (see make_image_spec in engine.cpp for the derivation per matrix).
 struct FrontColor {
-    int32_t sample;        // Vship_Sample_t
-    int32_t bits;          // 8..16 for integer samples, 1 for half/float
-    int32_t bytes;         // bytes per sample
-    int32_t range;         // Vship_Range_t
-    int32_t yuv;           // 1 when chroma planes carry the -0.5 / 128 offset
+    int32_t sample;        // SampleKind
+    uint32_t mask;         // bit mask of integer samples
     int32_t subw, subh;    // log2 chroma decimation
     int32_t h_cosited;     // horizontal chroma siting: 1 left, 0 centre
     int32_t v_cosited;     // vertical chroma siting: 1 top, 0 centre
-    int32_t matrix;        // Vship_YUVMatrix_t
     int32_t transfer;      // Vship_TransferFunction_t
-    int32_t use_prim;      // apply prim[] after linearisation
-    float prim[9];         // row-major 3x3, source primaries -> BT.709
+    int32_t use_pre;       // pre is not the identity
+    int32_t use_post;      // post is not the identity
+    float range0[3];       // scale, offset, gain of plane 0
+    float range12[3];      // scale, offset, gain of planes 1 and 2
+    float pre[9];          // row-major
+    float post[9];         // row-major
 };
 
 // Pyramid geometry of one pair (both images share it).
@@ -76,17 +82,33 @@ struct PairGeom {
     int64_t image_elems;                 // elements per image buffer
 };
 
-// One image of one pair for the front-end kernel.
+// One pair for the front-end kernel: both images of a tile are processed by
+// the same workgroup (the FP16 layout stores the distorted image as its
+// difference from the reference).
 struct FrontJob {
-    const uint8_t* plane[3];   // device raw planes
-    int64_t stride[3];         // bytes per row
-    void* out;                 // XYB planes of this image (StoreT elements)
-    int32_t width, height;     // level-0 size
-    int32_t tiles_x;           // 64-wide tiles along x
-    int32_t tile_start;        // first global tile index of this job
-    int32_t pair;              // index into the PairGeom table
-    int32_t color;             // index into the FrontColor table
+    const uint8_t* plane[2][3];  // device raw planes of image 0 (reference) and 1
+    int64_t stride[2][3];        // bytes per row
+    void* out[2];                // XYB planes of each image (StoreT elements)
+    int32_t color[2];            // index into the FrontColor table
+    int32_t width, height;       // level-0 size
+    int32_t tiles_x;             // 64-wide tiles along x
+    int32_t tile_start;          // first global tile index of this job
+    int32_t pair;                // index into the PairGeom table
+    int32_t pad;
 };
+
+// The stored XYB planes per scale (plane X, Y, B), fixed by the weight table:
+// a (plane, scale) is stored iff one of its three maps has an L1 or L4 weight
+// above 0.01 (reference src/ssimu2/score.hpp:4,122-131). The host derives the
+// same set from the weights and checks it against this table.
+constexpr bool kStored[5][3] = {
+    {false, true, false}, {true, true, false}, {true, true, false}, {true, true, true}, {true, true, true}};
+
+// FP16 layout: the reference image holds fp16(v - offset[plane]) and the
+// distorted image fp16(b - a) of the FP32 values. The difference keeps the
+// distortion exact to 11 bits wherever the images are close, and identical
+// inputs give a zero plane; the offsets centre X', Y', B' for finer steps.
+constexpr float kHalfOffset[3] = {0.42f, 0.5f, 0.55f};
 
 // One active (pair, plane, scale) for the blur/map kernel.
 struct MapJob {
