@@ -280,8 +280,8 @@ PairLayout make_pair_layout(int w, int h, int blur_config)
         for (int p = 0; p < 3; p++)
             if ((L.geom.off[sc][p] >= 0) != kStored[sc][p])
                 VSHIP_THROW(Vship_BadErrorType, "stored plane set differs from the kernels' table");
-    L.front_tiles_x = (w + kFrontTile - 1) / kFrontTile;
-    L.front_tiles_per_image = L.front_tiles_x * ((h + kFrontTile - 1) / kFrontTile);
+    L.front_tiles_x = (w + kFrontTileW - 1) / kFrontTileW;
+    L.front_tiles_per_image = L.front_tiles_x * ((h + kFrontTileH - 1) / kFrontTileH);
     int tiles = 0;
     for (int s = 0; s < kUsedScales; s++) {
         for (int p = 0; p < 3; p++) {

@@ -1,6 +1,6 @@
 // frontend.hip - fused colour front-end + 5-level pyramid + XYB for gfx950.
 //
-// One workgroup of 1024 threads owns a 64x64 level-0 tile of both images of a
+// One workgroup of 512 threads owns a 64x32 level-0 tile of both images of a
 // pair; thread (tx, ty) owns the 2x2 block at (2tx, 2ty). Per tile and image:
 //   raw samples -> range (rangeToFull.hpp:24-55) -> chroma Catmull-Rom
 //   upsampling through LDS (chromaUpsample.hpp:28-247) -> matrix, EOTF,
@@ -10,7 +10,7 @@
 //   edge clamp (downsample.hpp:3-19): level 1 in registers, 2 to 4 through
 //   LDS -> positive XYB (makeXYB.hpp:6-64) of exactly the planes the
 //   score uses.
-// A 64x64 level-0 tile maps onto whole 4x4 level-4 blocks, and the edge clamp
+// A 64x32 level-0 tile maps onto whole 4x2 level-4 blocks, and the edge clamp
 // of every level only reads pixels inside the same tile, so tiles are
 // independent. Both images' XYB values stay in registers until the tile is
 // stored, which the FP16 layout needs (distorted = fp16(b - a) of the FP32
@@ -28,6 +28,33 @@ namespace vship {
 namespace {
 
 VSHIP_INLINE int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Thread block of level-0 pixels: kBW columns x kBH rows per thread.
+constexpr int kBW = 2, kBH = 2;
+constexpr int kTX = kFrontTileW / kBW;  // threads along x (32)
+static_assert(kTX * (kFrontTileH / kBH) == kFrontThreads, "front-end tile and block shape");
+
+// Catmull-Rom weights of the four samples around segment parameter t: the
+// cubic Hermite form of resize.hpp:5-24 expanded (same polynomial).
+struct CrWeights {
+    float w0, w1, w2, w3;
+};
+
+VSHIP_INLINE CrWeights cr_weights(float t)
+{
+    const float t2 = t * t, t3 = t2 * t;
+    CrWeights w;
+    w.w0 = 0.5f * (-t3 + 2.0f * t2 - t);
+    w.w1 = 0.5f * (3.0f * t3 - 5.0f * t2 + 2.0f);
+    w.w2 = 0.5f * (-3.0f * t3 + 4.0f * t2 + t);
+    w.w3 = 0.5f * (t3 - t2);
+    return w;
+}
+
+VSHIP_INLINE float cr_apply(const CrWeights& w, float pm1, float p0, float p1, float p2)
+{
+    return fmaf(w.w0, pm1, fmaf(w.w1, p0, fmaf(w.w2, p1, w.w3 * p2)));
+}
 
 // Segment of full-resolution coordinate X for decimation 2^sub (sub >= 1):
 // X maps to segment [k, k+1] at parameter t. Cosited: sample k sits on
@@ -70,16 +97,16 @@ VSHIP_INLINE float load_kind(const uint8_t* p, uint32_t mask)
 template <int KIND>
 constexpr int kind_bytes() { return KIND == kSampleU8 ? 1 : (KIND == kSampleFloat ? 4 : 2); }
 
-// 2x2 block of one plane at full resolution (clamped at the image edge).
+// kBH x kBW block of one plane at full resolution (clamped at the image edge).
 template <int KIND>
 VSHIP_INLINE void load_block(const uint8_t* base, int64_t stride, int x0, int y0, int W, int H, uint32_t mask,
-                             const RangeXform& rx, float out[2][2])
+                             const RangeXform& rx, float out[kBH][kBW])
 {
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
+    for (int j = 0; j < kBH; j++) {
         const uint8_t* row = base + (int64_t)min(y0 + j, H - 1) * stride;
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+        for (int i = 0; i < kBW; i++)
             out[j][i] = apply_range(load_kind<KIND>(row + min(x0 + i, W - 1) * kind_bytes<KIND>(), mask), rx);
     }
 }
@@ -91,34 +118,34 @@ VSHIP_INLINE void stage_chroma(const uint8_t* const* plane, const int64_t* pstri
                                int ch)
 {
     const int t = threadIdx.x;
-    const int tx = t & 31, ty = t >> 5;
+    const int tx = t % kTX, ty = t / kTX;
 #pragma unroll
     for (int p = 0; p < 2; p++) {
         const uint8_t* base = plane[1 + p];
         const int64_t stride = pstride[1 + p];
         float* dst = s_chroma + p * ccols * crows;
-        for (int r = ty; r < crows; r += 32) {
+        for (int r = ty; r < crows; r += kFrontThreads / kTX) {
             const uint8_t* row = base + (int64_t)clampi(cy0 + r, 0, ch - 1) * stride;
-            for (int c = tx; c < ccols; c += 32)
+            for (int c = tx; c < ccols; c += kTX)
                 dst[r * ccols + c] =
                     apply_range(load_kind<KIND>(row + clampi(cx0 + c, 0, cw - 1) * kind_bytes<KIND>(), mask), rx);
         }
     }
 }
 
-// EOTF of the whole 2x2x3 block, one code path per transfer.
+// EOTF of the thread's block, one code path per transfer.
 template <int TRC>
-VSHIP_INLINE void eotf_block(float P[3][2][2])
+VSHIP_INLINE void eotf_block(float P[3][kBH][kBW])
 {
 #pragma unroll
     for (int c = 0; c < 3; c++)
 #pragma unroll
-        for (int j = 0; j < 2; j++)
+        for (int j = 0; j < kBH; j++)
 #pragma unroll
-            for (int i = 0; i < 2; i++) P[c][j][i] = eotf(P[c][j][i], TRC);
+            for (int i = 0; i < kBW; i++) P[c][j][i] = eotf(P[c][j][i], TRC);
 }
 
-VSHIP_INLINE void apply_transfer(int trc, float P[3][2][2])
+VSHIP_INLINE void apply_transfer(int trc, float P[3][kBH][kBW])
 {
     switch (trc) {
     case 1: eotf_block<1>(P); break;
@@ -130,9 +157,9 @@ VSHIP_INLINE void apply_transfer(int trc, float P[3][2][2])
     case 17: eotf_block<17>(P); break;
     case 18:
 #pragma unroll
-        for (int j = 0; j < 2; j++)
+        for (int j = 0; j < kBH; j++)
 #pragma unroll
-            for (int i = 0; i < 2; i++) eotf_hlg(P[0][j][i], P[1][j][i], P[2][j][i]);
+            for (int i = 0; i < kBW; i++) eotf_hlg(P[0][j][i], P[1][j][i], P[2][j][i]);
         break;
     default: break;  // linear
     }
@@ -140,26 +167,29 @@ VSHIP_INLINE void apply_transfer(int trc, float P[3][2][2])
 
 // XYB values of one image's tile in the thread's ownership (stored planes only).
 struct TileXyb {
-    float y0[2][2];   // level 0: Y of the 2x2 block
-    float x1, y1;     // level 1: X, Y of one pixel
-    float x2, y2;     // level 2 (threads < 256)
+    float y0[kBH][kBW];              // level 0: Y of the thread's block
+    float x1[kBH / 2], y1[kBH / 2];  // level 1: X, Y of the block's parents
+    float x2, y2;                    // level 2 (threads < 256)
     float l3[3];      // level 3 (threads < 64): X, Y, B
     float l4[3];      // level 4 (threads < 16): X, Y, B
 };
 
 // Level s+1 pixel from the 2x2 parents at (2lx, 2ly) of a level stored in LDS
-// as [3][n][n + 1], with the edge clamp min(2x+1, w-1) (downsample.hpp:3-19).
-VSHIP_INLINE void down_lds(const float* src, int n, int lx, int ly, int gx2, int gy2, int w, int h, float out[3])
+// as [3][rows][cols + 1], with the edge clamp min(2x+1, w-1)
+// (downsample.hpp:3-19); (gx2, gy2) are the parents' global coordinates.
+VSHIP_INLINE void down_lds(const float* src, int cols, int rows, int lx, int ly, int gx2, int gy2, int w, int h,
+                           float out[3])
 {
+    const int pitch = cols + 1;
     const int xb = (gx2 + 1 < w) ? 2 * lx + 1 : 2 * lx;
     const int yb = (gy2 + 1 < h) ? 2 * ly + 1 : 2 * ly;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        const float* b = src + c * n * (n + 1);
-        float s = b[(2 * ly) * (n + 1) + 2 * lx];
-        s += b[yb * (n + 1) + 2 * lx];
-        s += b[(2 * ly) * (n + 1) + xb];
-        s += b[yb * (n + 1) + xb];
+        const float* b = src + c * rows * pitch;
+        float s = b[(2 * ly) * pitch + 2 * lx];
+        s += b[yb * pitch + 2 * lx];
+        s += b[(2 * ly) * pitch + xb];
+        s += b[yb * pitch + xb];
         out[c] = s * 0.25f;
     }
 }
@@ -170,14 +200,14 @@ VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc,
     const uint8_t* const* plane = job.plane[k];
     const int64_t* pstride = job.stride[k];
     const int t = threadIdx.x;
-    const int tx = t & 31, ty = t >> 5;
+    const int tx = t % kTX, ty = t / kTX;
     const int W = job.width, H = job.height;
-    const int x0 = tile_x * kFrontTile + 2 * tx;
-    const int y0 = tile_y * kFrontTile + 2 * ty;
+    const int x0 = tile_x * kFrontTileW + kBW * tx;
+    const int y0 = tile_y * kFrontTileH + kBH * ty;
     const RangeXform r0 = {fc.range0[0], fc.range0[1], fc.range0[2]};
     const RangeXform r12 = {fc.range12[0], fc.range12[1], fc.range12[2]};
 
-    float P[3][2][2];  // plane values, then linear R, G, B of the 2x2 block
+    float P[3][kBH][kBW];  // plane values, then linear R, G, B of the block
     const bool sub = fc.subw || fc.subh;
     switch (fc.sample) {
     case kSampleU8:
@@ -214,10 +244,10 @@ VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc,
         // both chroma planes staged in LDS with a 2-sample halo; clamped
         // coordinates reproduce the reference's edge rules
         // (chromaUpsample.hpp:29-35,41-47)
-        const int ccols = (kFrontTile >> fc.subw) + 4;
-        const int crows = (kFrontTile >> fc.subh) + 4;
-        const int cx0 = ((tile_x * kFrontTile) >> fc.subw) - 2;
-        const int cy0 = ((tile_y * kFrontTile) >> fc.subh) - 2;
+        const int ccols = (kFrontTileW >> fc.subw) + 4;
+        const int crows = (kFrontTileH >> fc.subh) + 4;
+        const int cx0 = ((tile_x * kFrontTileW) >> fc.subw) - 2;
+        const int cy0 = ((tile_y * kFrontTileH) >> fc.subh) - 2;
         const int cw = ((W - 1) >> fc.subw) + 1;
         const int ch = ((H - 1) >> fc.subh) + 1;
         const int plane_elems = ccols * crows;
@@ -234,56 +264,64 @@ VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc,
         if (both) {
             // horizontal pass over every staged chroma row first, like the
             // reference (chromaUpsample.hpp:165-202)
-            const int X = t & (kFrontTile - 1);
+            const int X = t & (kFrontTileW - 1);
             int kk; float tt; bool copy;
-            seg_of(tile_x * kFrontTile + X, fc.subw, fc.h_cosited != 0, kk, tt, copy);
+            seg_of(tile_x * kFrontTileW + X, fc.subw, fc.h_cosited != 0, kk, tt, copy);
             const int c = kk - cx0;
+            const CrWeights cw4 = cr_weights(tt);
 #pragma unroll
             for (int p = 0; p < 2; p++) {
-                for (int r = t / kFrontTile; r < crows; r += kFrontThreads / kFrontTile) {
+                for (int r = t / kFrontTileW; r < crows; r += kFrontThreads / kFrontTileW) {
                     const float* row = s_chroma + p * plane_elems + r * ccols;
-                    s_mid[(p * crows + r) * kFrontTile + X] =
-                        copy ? row[c] : catmull_rom(row[c - 1], row[c], row[c + 1], row[c + 2], tt);
+                    s_mid[(p * crows + r) * kFrontTileW + X] =
+                        copy ? row[c] : cr_apply(cw4, row[c - 1], row[c], row[c + 1], row[c + 2]);
                 }
             }
             __syncthreads();
         }
-        // segment parameters of the thread's 2 columns and 2 rows
-        int kx[2], ky[2];
-        float txp[2], typ[2];
-        bool cpx[2], cpy[2];
+        // segment parameters of the thread's columns and rows
+        int kx[kBW], ky[kBH];
+        CrWeights wx[kBW], wy[kBH];
+        bool cpx[kBW], cpy[kBH];
 #pragma unroll
-        for (int i = 0; i < 2; i++) {
-            if (fc.subw) seg_of(x0 + i, fc.subw, fc.h_cosited != 0, kx[i], txp[i], cpx[i]);
-            else { kx[i] = x0 + i; txp[i] = 0.0f; cpx[i] = true; }
-            if (fc.subh) seg_of(y0 + i, fc.subh, fc.v_cosited != 0, ky[i], typ[i], cpy[i]);
-            else { ky[i] = y0 + i; typ[i] = 0.0f; cpy[i] = true; }
+        for (int i = 0; i < kBW; i++) {
+            float tt = 0.0f;
+            if (fc.subw) seg_of(x0 + i, fc.subw, fc.h_cosited != 0, kx[i], tt, cpx[i]);
+            else { kx[i] = x0 + i; cpx[i] = true; }
+            wx[i] = cr_weights(tt);
+        }
+#pragma unroll
+        for (int j = 0; j < kBH; j++) {
+            float tt = 0.0f;
+            if (fc.subh) seg_of(y0 + j, fc.subh, fc.v_cosited != 0, ky[j], tt, cpy[j]);
+            else { ky[j] = y0 + j; cpy[j] = true; }
+            wy[j] = cr_weights(tt);
         }
 #pragma unroll
         for (int p = 0; p < 2; p++) {
 #pragma unroll
-            for (int j = 0; j < 2; j++) {
+            for (int j = 0; j < kBH; j++) {
 #pragma unroll
-                for (int i = 0; i < 2; i++) {
+                for (int i = 0; i < kBW; i++) {
                     float val;
                     if (fc.subh == 0) {
                         const float* row = s_chroma + p * plane_elems + (y0 + j - cy0) * ccols;
                         const int c = kx[i] - cx0;
-                        val = cpx[i] ? row[c] : catmull_rom(row[c - 1], row[c], row[c + 1], row[c + 2], txp[i]);
+                        val = cpx[i] ? row[c] : cr_apply(wx[i], row[c - 1], row[c], row[c + 1], row[c + 2]);
                     } else {
                         const int r = ky[j] - cy0;
                         const float* colbase;
                         int pitch;
                         if (both) {
-                            colbase = s_mid + p * crows * kFrontTile + (x0 + i - tile_x * kFrontTile);
-                            pitch = kFrontTile;
+                            colbase = s_mid + p * crows * kFrontTileW + (x0 + i - tile_x * kFrontTileW);
+                            pitch = kFrontTileW;
                         } else {
                             colbase = s_chroma + p * plane_elems + (x0 + i - cx0);
                             pitch = ccols;
                         }
                         val = cpy[j] ? colbase[r * pitch]
-                                     : catmull_rom(colbase[(r - 1) * pitch], colbase[r * pitch],
-                                                   colbase[(r + 1) * pitch], colbase[(r + 2) * pitch], typ[j]);
+                                     : cr_apply(wy[j], colbase[(r - 1) * pitch], colbase[r * pitch],
+                                                colbase[(r + 1) * pitch], colbase[(r + 2) * pitch]);
                     }
                     P[1 + p][j][i] = val;
                 }
@@ -294,79 +332,82 @@ VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc,
     // matrix, EOTF, linear-light matrix with primaries (P becomes linear RGB)
     if (fc.use_pre) {
 #pragma unroll
-        for (int j = 0; j < 2; j++)
+        for (int j = 0; j < kBH; j++)
 #pragma unroll
-            for (int i = 0; i < 2; i++) mat3_apply(fc.pre, P[0][j][i], P[1][j][i], P[2][j][i]);
+            for (int i = 0; i < kBW; i++) mat3_apply(fc.pre, P[0][j][i], P[1][j][i], P[2][j][i]);
     }
     apply_transfer(fc.transfer, P);
     if (fc.use_post) {
 #pragma unroll
-        for (int j = 0; j < 2; j++)
+        for (int j = 0; j < kBH; j++)
 #pragma unroll
-            for (int i = 0; i < 2; i++) mat3_apply(fc.post, P[0][j][i], P[1][j][i], P[2][j][i]);
+            for (int i = 0; i < kBW; i++) mat3_apply(fc.post, P[0][j][i], P[1][j][i], P[2][j][i]);
     }
 
     // ---- level 0: Y only
 #pragma unroll
-    for (int j = 0; j < 2; j++)
+    for (int j = 0; j < kBH; j++)
 #pragma unroll
-        for (int i = 0; i < 2; i++) res.y0[j][i] = to_xyb(P[0][j][i], P[1][j][i], P[2][j][i], false).y;
+        for (int i = 0; i < kBW; i++) res.y0[j][i] = to_xyb(P[0][j][i], P[1][j][i], P[2][j][i], false).y;
 
-    // ---- level 1 (1 per thread), edge clamp min(2x+1, w-1)
-    float L1[3];
-    {
-        const bool ylast = !(y0 + 1 < H);
-        const bool xlast = !(x0 + 1 < W);
+    // ---- level 1 (kBH/2 rows of one column per thread), edge clamp min(2x+1, w-1)
+    float L1[kBH / 2][3];
+    const bool xlast = !(x0 + 1 < W);
+#pragma unroll
+    for (int j = 0; j < kBH / 2; j++) {
+        const bool ylast = !(y0 + 2 * j + 1 < H);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const float p00 = P[c][0][0];
-            const float p10 = ylast ? p00 : P[c][1][0];
-            const float p01 = xlast ? p00 : P[c][0][1];
-            const float p11 = ylast ? p01 : (xlast ? p10 : P[c][1][1]);
+            const float p00 = P[c][2 * j][0];
+            const float p10 = ylast ? p00 : P[c][2 * j + 1][0];
+            const float p01 = xlast ? p00 : P[c][2 * j][1];
+            const float p11 = ylast ? p01 : (xlast ? p10 : P[c][2 * j + 1][1]);
             float s = p00;
             s += p10;
             s += p01;
             s += p11;
-            L1[c] = s * 0.25f;
+            L1[j][c] = s * 0.25f;
         }
-        const XYB q = to_xyb(L1[0], L1[1], L1[2], false);
-        res.x1 = q.x;
-        res.y1 = q.y;
+        const XYB q = to_xyb(L1[j][0], L1[j][1], L1[j][2], false);
+        res.x1[j] = q.x;
+        res.y1[j] = q.y;
     }
 
-    // ---- levels 2, 3, 4 through LDS (32x32 -> 16x16 -> 8x8 -> 4x4 per tile)
-    float* s_l1 = lds;                       // [3][32][33]
-    float* s_l2 = s_l1 + 3 * 32 * 33;        // [3][16][17]
-    float* s_l3 = s_l2 + 3 * 16 * 17;        // [3][8][9]
+    // ---- levels 2, 3, 4 through LDS (32x16 -> 16x8 -> 8x4 -> 4x2 per tile)
+    float* s_l1 = lds;                       // [3][16][33]
+    float* s_l2 = s_l1 + 3 * 16 * 33;        // [3][8][17]
+    float* s_l3 = s_l2 + 3 * 8 * 17;         // [3][4][9]
     __syncthreads();                         // chroma staging no longer needed
 #pragma unroll
-    for (int c = 0; c < 3; c++) s_l1[c * 32 * 33 + ty * 33 + tx] = L1[c];
+    for (int j = 0; j < kBH / 2; j++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) s_l1[(c * 16 + ty * (kBH / 2) + j) * 33 + tx] = L1[j][c];
     __syncthreads();
-    if (t < 256) {
+    if (t < 128) {
         const int lx = t & 15, ly = t >> 4;
         float L2[3];
-        down_lds(s_l1, 32, lx, ly, tile_x * 32 + 2 * lx, tile_y * 32 + 2 * ly, geo.w[1], geo.h[1], L2);
+        down_lds(s_l1, 32, 16, lx, ly, tile_x * 32 + 2 * lx, tile_y * 16 + 2 * ly, geo.w[1], geo.h[1], L2);
 #pragma unroll
-        for (int c = 0; c < 3; c++) s_l2[c * 16 * 17 + ly * 17 + lx] = L2[c];
+        for (int c = 0; c < 3; c++) s_l2[(c * 8 + ly) * 17 + lx] = L2[c];
         const XYB q = to_xyb(L2[0], L2[1], L2[2], false);
         res.x2 = q.x;
         res.y2 = q.y;
     }
     __syncthreads();
-    if (t < 64) {
+    if (t < 32) {
         const int lx = t & 7, ly = t >> 3;
         float L3[3];
-        down_lds(s_l2, 16, lx, ly, tile_x * 16 + 2 * lx, tile_y * 16 + 2 * ly, geo.w[2], geo.h[2], L3);
+        down_lds(s_l2, 16, 8, lx, ly, tile_x * 16 + 2 * lx, tile_y * 8 + 2 * ly, geo.w[2], geo.h[2], L3);
 #pragma unroll
-        for (int c = 0; c < 3; c++) s_l3[c * 8 * 9 + ly * 9 + lx] = L3[c];
+        for (int c = 0; c < 3; c++) s_l3[(c * 4 + ly) * 9 + lx] = L3[c];
         const XYB q = to_xyb(L3[0], L3[1], L3[2], true);
         res.l3[0] = q.x; res.l3[1] = q.y; res.l3[2] = q.b;
     }
     __syncthreads();
-    if (t < 16) {
+    if (t < 8) {
         const int lx = t & 3, ly = t >> 2;
         float L4[3];
-        down_lds(s_l3, 8, lx, ly, tile_x * 8 + 2 * lx, tile_y * 8 + 2 * ly, geo.w[3], geo.h[3], L4);
+        down_lds(s_l3, 8, 4, lx, ly, tile_x * 8 + 2 * lx, tile_y * 4 + 2 * ly, geo.w[3], geo.h[3], L4);
         const XYB q = to_xyb(L4[0], L4[1], L4[2], true);
         res.l4[0] = q.x; res.l4[1] = q.y; res.l4[2] = q.b;
     }
@@ -393,42 +434,46 @@ VSHIP_INLINE void store_pair(const FrontJob& job, const PairGeom& geo, int tile_
                              const TileXyb& b)
 {
     const int t = threadIdx.x;
-    const int tx = t & 31, ty = t >> 5;
+    const int tx = t % kTX, ty = t / kTX;
     const int W = job.width, H = job.height;
-    const int x0 = tile_x * kFrontTile + 2 * tx;
-    const int y0 = tile_y * kFrontTile + 2 * ty;
+    const int x0 = tile_x * kFrontTileW + kBW * tx;
+    const int y0 = tile_y * kFrontTileH + kBH * ty;
     StoreT* o0 = static_cast<StoreT*>(job.out[0]);
     StoreT* o1 = static_cast<StoreT*>(job.out[1]);
 #pragma unroll
-    for (int j = 0; j < 2; j++)
+    for (int j = 0; j < kBH; j++)
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+        for (int i = 0; i < kBW; i++)
             if (y0 + j < H && x0 + i < W)
                 put(o0, o1, geo.off[0][1] + (int64_t)(y0 + j) * geo.pitch[0] + x0 + i, 1, a.y0[j][i], b.y0[j][i]);
-    const int x1 = tile_x * 32 + tx, y1 = tile_y * 32 + ty;
-    if (y1 < geo.h[1] && x1 < geo.w[1]) {
-        const int64_t o = (int64_t)y1 * geo.pitch[1] + x1;
-        put(o0, o1, geo.off[1][0] + o, 0, a.x1, b.x1);
-        put(o0, o1, geo.off[1][1] + o, 1, a.y1, b.y1);
+    const int x1 = tile_x * 32 + tx;
+#pragma unroll
+    for (int j = 0; j < kBH / 2; j++) {
+        const int y1 = tile_y * 16 + ty * (kBH / 2) + j;
+        if (y1 < geo.h[1] && x1 < geo.w[1]) {
+            const int64_t o = (int64_t)y1 * geo.pitch[1] + x1;
+            put(o0, o1, geo.off[1][0] + o, 0, a.x1[j], b.x1[j]);
+            put(o0, o1, geo.off[1][1] + o, 1, a.y1[j], b.y1[j]);
+        }
     }
-    if (t < 256) {
-        const int x2 = tile_x * 16 + (t & 15), y2 = tile_y * 16 + (t >> 4);
+    if (t < 128) {
+        const int x2 = tile_x * 16 + (t & 15), y2 = tile_y * 8 + (t >> 4);
         if (x2 < geo.w[2] && y2 < geo.h[2]) {
             const int64_t o = (int64_t)y2 * geo.pitch[2] + x2;
             put(o0, o1, geo.off[2][0] + o, 0, a.x2, b.x2);
             put(o0, o1, geo.off[2][1] + o, 1, a.y2, b.y2);
         }
     }
-    if (t < 64) {
-        const int x3 = tile_x * 8 + (t & 7), y3 = tile_y * 8 + (t >> 3);
+    if (t < 32) {
+        const int x3 = tile_x * 8 + (t & 7), y3 = tile_y * 4 + (t >> 3);
         if (x3 < geo.w[3] && y3 < geo.h[3]) {
             const int64_t o = (int64_t)y3 * geo.pitch[3] + x3;
 #pragma unroll
             for (int c = 0; c < 3; c++) put(o0, o1, geo.off[3][c] + o, c, a.l3[c], b.l3[c]);
         }
     }
-    if (t < 16) {
-        const int x4 = tile_x * 4 + (t & 3), y4 = tile_y * 4 + (t >> 2);
+    if (t < 8) {
+        const int x4 = tile_x * 4 + (t & 3), y4 = tile_y * 2 + (t >> 2);
         if (x4 < geo.w[4] && y4 < geo.h[4]) {
             const int64_t o = (int64_t)y4 * geo.pitch[4] + x4;
 #pragma unroll
@@ -438,7 +483,7 @@ VSHIP_INLINE void store_pair(const FrontJob& job, const PairGeom& geo, int tile_
 }
 
 template <typename StoreT>
-__global__ __launch_bounds__(kFrontThreads)
+__global__ __launch_bounds__(kFrontThreads, 4)
 void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ counts,
                      const FrontColor* __restrict__ colors, const PairGeom* __restrict__ geoms)
 {
@@ -470,11 +515,11 @@ size_t frontend_lds_bytes(const FrontColor& fc)
 {
     size_t chroma = 0;
     if (fc.subw || fc.subh) {
-        const size_t ccols = (kFrontTile >> fc.subw) + 4, crows = (kFrontTile >> fc.subh) + 4;
+        const size_t ccols = (kFrontTileW >> fc.subw) + 4, crows = (kFrontTileH >> fc.subh) + 4;
         chroma = 2 * ccols * crows;
-        if (fc.subw && fc.subh) chroma += 2 * crows * kFrontTile;
+        if (fc.subw && fc.subh) chroma += 2 * crows * kFrontTileW;
     }
-    const size_t levels = 3 * 32 * 33 + 3 * 16 * 17 + 3 * 8 * 9;
+    const size_t levels = 3 * 16 * 33 + 3 * 8 * 17 + 3 * 4 * 9;
     return sizeof(float) * (chroma > levels ? chroma : levels);
 }
 

@@ -20,9 +20,11 @@ constexpr int kUsedScales = 5;    // scale 5 carries no active map
 constexpr int kMaxJobsPerPair = 11;
 constexpr float kC2 = 0.0009f;    // SSIM constant (score.hpp:272-273)
 
-// Front-end tile: 64x64 level-0 pixels, 1024 threads, each owning a 2x2 block.
-constexpr int kFrontTile = 64;
-constexpr int kFrontThreads = 1024;
+// Front-end tile: 64x32 level-0 pixels (whole 4x2 blocks at level 4), 512
+// threads, each owning a 2x2 block.
+constexpr int kFrontTileW = 64;
+constexpr int kFrontTileH = 32;
+constexpr int kFrontThreads = 512;
 
 // Blur/map tiles: 256 threads, output tile shape chosen per launch (blur.hip),
 // halo 8 on every side.
