@@ -27,7 +27,6 @@ namespace vship {
 
 namespace {
 
-constexpr int kXcds = 8;
 
 __device__ inline float inv_norm(int x, int w, const BlurTaps& taps)
 {

@@ -3,10 +3,10 @@
 // Float restatements of the reference's colour steps; every function cites the
 // reference file:line it follows. Divisions by constants are done as
 // multiplications by the reciprocal and powers/cube roots use the hardware
-// log2/exp2 with one Newton step for the cube root: each differs from the
-// reference's IEEE-division / libm result by a few ulp, far below the
-// |delta score| <= 0.005 bar, and both images of a pair go through the same
-// code, so identical inputs still give identical planes.
+// log2/exp2: each differs from the reference's IEEE-division / libm result by
+// a few ulp, far below the |delta score| <= 0.005 bar, and both images of a
+// pair go through the same code, so identical inputs still give identical
+// planes.
 #pragma once
 
 #include "ssimu2_common.hpp"
@@ -15,33 +15,38 @@ namespace vship {
 
 enum SampleKind : int32_t { kSampleU8 = 0, kSampleU16 = 1, kSampleHalf = 2, kSampleFloat = 3 };
 
-// Range normalisation constants of one plane, reference
+// Range normalisation of one plane, reference
 // src/gpuColorToLinear/rangeToFull.hpp:24-55: v' = (v * scale - offset) * gain
 // full range:    scale = 1/(2^b - 1), offset = 0.5 (YUV chroma) or 0, gain = 1
 // limited range: scale = 2^-(b-8) (x256 for half/float), offset 16 / 128,
 //                gain 1/219 (luma) or 1/224 (chroma)
+// folded on the host (in double) into one fused multiply-add
+// v' = v * mul + add with mul = scale * gain, add = -offset * gain.
 struct RangeXform {
-    float scale, offset, gain;
+    float mul, add;
 };
 
-__host__ __device__ inline RangeXform make_range(int range, int bits, bool chroma)
+inline RangeXform make_range(int range, int bits, bool chroma)
 {
-    RangeXform r;
+    double scale, offset, gain;
     if (range == 1) {
-        r.scale = 1.0f / (float)((1 << bits) - 1);
-        r.offset = chroma ? 0.5f : 0.0f;
-        r.gain = 1.0f;
+        scale = 1.0 / (double)((1 << bits) - 1);
+        offset = chroma ? 0.5 : 0.0;
+        gain = 1.0;
     } else {
-        r.scale = bits >= 8 ? 1.0f / (float)(1 << (bits - 8)) : 256.0f;
-        r.offset = chroma ? 128.0f : 16.0f;
-        r.gain = chroma ? 1.0f / 224.0f : 1.0f / 219.0f;
+        scale = bits >= 8 ? 1.0 / (double)(1 << (bits - 8)) : 256.0;
+        offset = chroma ? 128.0 : 16.0;
+        gain = chroma ? 1.0 / 224.0 : 1.0 / 219.0;
     }
+    RangeXform r;
+    r.mul = (float)(scale * gain);
+    r.add = (float)(-offset * gain);
     return r;
 }
 
 __device__ inline float apply_range(float v, const RangeXform& r)
 {
-    return (v * r.scale - r.offset) * r.gain;
+    return fmaf(v, r.mul, r.add);
 }
 
 // Cubic Hermite segment with Catmull-Rom tangents, reference
@@ -70,15 +75,13 @@ __device__ inline float pow_signed(float x, float g)
     return x < 0.0f ? -pow_pos(-x, g) : pow_pos(x, g);
 }
 
-// Cube root of x >= 0: exp2(log2(x)/3) refined by one Newton step
-// y <- (2y + x/y^2)/3.
+// Cube root of x >= 0 as exp2(log2(x) / 3) from the hardware log2/exp2
+// (each about 1 ulp): relative error of a few 1e-7 over the range the opsin
+// mix produces, far below what the score can resolve.
 __device__ inline float cbrt_pos(float x)
 {
     if (!(x > 0.0f)) return 0.0f;
-    float y = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.0f / 3.0f));
-    const float r = __builtin_amdgcn_rcpf(y * y);
-    y = (2.0f * y + x * r) * (1.0f / 3.0f);
-    return y;
+    return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.0f / 3.0f));
 }
 
 // Per-component EOTFs, reference src/gpuColorToLinear/transferToLinear.hpp:
@@ -131,8 +134,9 @@ __device__ inline void mat3_apply(const float m[9], float& x, float& y, float& z
     x = a; y = b; z = c;
 }
 
-// Linear RGB -> positive XYB, reference src/ssimu2/makeXYB.hpp:6-52. The last
-// three additions use double constants like the reference (:44-46).
+// Linear RGB -> positive XYB, reference src/ssimu2/makeXYB.hpp:6-52. The
+// reference adds the last three offsets in double (:44-46); float additions
+// here differ by at most one ulp of the stored value.
 struct XYB {
     float x, y, b;
 };
@@ -148,12 +152,12 @@ __device__ inline XYB to_xyb(float r, float g, float b, bool need_b)
     XYB o;
     const float X = 0.5f * (l - m);
     const float Y = X + m;
-    o.x = (float)((double)(X * 14.0f) + 0.42);
-    o.y = (float)((double)Y + 0.01);
+    o.x = fmaf(X, 14.0f, 0.42f);
+    o.y = Y + 0.01f;
     if (need_b) {
         float s = fmaf(0.24342269f, r, fmaf(0.20476745f, g, fmaf(0.55180986f, b, bias)));
         s = cbrt_pos(s) + cbias;
-        o.b = (float)((double)(s - Y) + 0.55);
+        o.b = (s - Y) + 0.55f;
     } else {
         o.b = 0.0f;
     }

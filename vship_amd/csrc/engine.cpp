@@ -212,8 +212,8 @@ ImageSpec make_image_spec(const Vship_Colorspace_t& cs)
     fc.transfer = cs.transferFunction;
     const RangeXform r0 = make_range(cs.range, bits, false);
     const RangeXform r12 = make_range(cs.range, bits, cs.colorFamily == Vship_ColorYUV);
-    fc.range0[0] = r0.scale; fc.range0[1] = r0.offset; fc.range0[2] = r0.gain;
-    fc.range12[0] = r12.scale; fc.range12[1] = r12.offset; fc.range12[2] = r12.gain;
+    fc.range0[0] = r0.mul; fc.range0[1] = r0.add;
+    fc.range12[0] = r12.mul; fc.range12[1] = r12.add;
     // the reference skips the whole matrix/EOTF step for linear RGB (YUVToLinRGB.hpp:219)
     const int matrix = (cs.transferFunction == Vship_TRC_Linear && cs.YUVMatrix == Vship_MATRIX_RGB) ? Vship_MATRIX_RGB
                                                                                                       : cs.YUVMatrix;

@@ -169,10 +169,33 @@ VSHIP_INLINE void apply_transfer(int trc, float P[3][kBH][kBW])
 struct TileXyb {
     float y0[kBH][kBW];              // level 0: Y of the thread's block
     float x1[kBH / 2], y1[kBH / 2];  // level 1: X, Y of the block's parents
-    float x2, y2;                    // level 2 (threads < 256)
-    float l3[3];      // level 3 (threads < 64): X, Y, B
-    float l4[3];      // level 4 (threads < 16): X, Y, B
+    float x2 = 0.0f, y2 = 0.0f;      // level 2 (threads < 128)
+    float l3[3] = {};                // level 3 (threads < 32): X, Y, B
+    float l4[3] = {};                // level 4 (threads < 8): X, Y, B
 };
+
+// Flat field index of a TileXyb value (the FP16 path's LDS stash is
+// field-major).
+constexpr int kFieldY0 = 0;
+constexpr int kFieldX1 = kFieldY0 + kBH * kBW;
+constexpr int kFieldY1 = kFieldX1 + kBH / 2;
+constexpr int kFieldX2 = kFieldY1 + kBH / 2;
+constexpr int kFieldY2 = kFieldX2 + 1;
+constexpr int kFieldL3 = kFieldY2 + 1;
+constexpr int kFieldL4 = kFieldL3 + 3;
+constexpr int kFields = kFieldL4 + 3;
+static_assert(kFields * kFrontThreads == kFrontStashFloats, "front-end stash size");
+
+VSHIP_INLINE float field(const TileXyb& r, int f)
+{
+    if (f < kFieldX1) return r.y0[(f - kFieldY0) / kBW][(f - kFieldY0) % kBW];
+    if (f < kFieldY1) return r.x1[f - kFieldX1];
+    if (f < kFieldX2) return r.y1[f - kFieldY1];
+    if (f == kFieldX2) return r.x2;
+    if (f == kFieldY2) return r.y2;
+    if (f < kFieldL4) return r.l3[f - kFieldL3];
+    return r.l4[f - kFieldL4];
+}
 
 // Level s+1 pixel from the 2x2 parents at (2lx, 2ly) of a level stored in LDS
 // as [3][rows][cols + 1], with the edge clamp min(2x+1, w-1)
@@ -204,8 +227,8 @@ VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc,
     const int W = job.width, H = job.height;
     const int x0 = tile_x * kFrontTileW + kBW * tx;
     const int y0 = tile_y * kFrontTileH + kBH * ty;
-    const RangeXform r0 = {fc.range0[0], fc.range0[1], fc.range0[2]};
-    const RangeXform r12 = {fc.range12[0], fc.range12[1], fc.range12[2]};
+    const RangeXform r0 = {fc.range0[0], fc.range0[1]};
+    const RangeXform r12 = {fc.range12[0], fc.range12[1]};
 
     float P[3][kBH][kBW];  // plane values, then linear R, G, B of the block
     const bool sub = fc.subw || fc.subh;
@@ -414,54 +437,38 @@ VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc,
     __syncthreads();  // LDS is reused by the next image
 }
 
-// Stored value of a plane element: FP32 planes hold the values themselves;
-// FP16 planes hold fp16(a - offset) for the reference and fp16(b - a) for the
-// distorted image.
-template <typename StoreT>
-VSHIP_INLINE void put(StoreT* out0, StoreT* out1, int64_t o, int plane, float a, float b)
-{
-    if constexpr (sizeof(StoreT) == 4) {
-        out0[o] = store_cast<StoreT>(a);
-        out1[o] = store_cast<StoreT>(b);
-    } else {
-        out0[o] = store_cast<StoreT>(a - kHalfOffset[plane]);
-        out1[o] = store_cast<StoreT>(b - a);
-    }
-}
-
-template <typename StoreT>
-VSHIP_INLINE void store_pair(const FrontJob& job, const PairGeom& geo, int tile_x, int tile_y, const TileXyb& a,
-                             const TileXyb& b)
+// Writes the tile's stored planes. emit(plane, element offset, k) writes
+// plane element `off` from field value k of the thread's TileXyb.
+template <typename Emit>
+VSHIP_INLINE void store_tile(const FrontJob& job, const PairGeom& geo, int tile_x, int tile_y, Emit emit)
 {
     const int t = threadIdx.x;
     const int tx = t % kTX, ty = t / kTX;
     const int W = job.width, H = job.height;
     const int x0 = tile_x * kFrontTileW + kBW * tx;
     const int y0 = tile_y * kFrontTileH + kBH * ty;
-    StoreT* o0 = static_cast<StoreT*>(job.out[0]);
-    StoreT* o1 = static_cast<StoreT*>(job.out[1]);
 #pragma unroll
     for (int j = 0; j < kBH; j++)
 #pragma unroll
         for (int i = 0; i < kBW; i++)
             if (y0 + j < H && x0 + i < W)
-                put(o0, o1, geo.off[0][1] + (int64_t)(y0 + j) * geo.pitch[0] + x0 + i, 1, a.y0[j][i], b.y0[j][i]);
+                emit(1, geo.off[0][1] + (int64_t)(y0 + j) * geo.pitch[0] + x0 + i, kFieldY0 + j * kBW + i);
     const int x1 = tile_x * 32 + tx;
 #pragma unroll
     for (int j = 0; j < kBH / 2; j++) {
         const int y1 = tile_y * 16 + ty * (kBH / 2) + j;
         if (y1 < geo.h[1] && x1 < geo.w[1]) {
             const int64_t o = (int64_t)y1 * geo.pitch[1] + x1;
-            put(o0, o1, geo.off[1][0] + o, 0, a.x1[j], b.x1[j]);
-            put(o0, o1, geo.off[1][1] + o, 1, a.y1[j], b.y1[j]);
+            emit(0, geo.off[1][0] + o, kFieldX1 + j);
+            emit(1, geo.off[1][1] + o, kFieldY1 + j);
         }
     }
     if (t < 128) {
         const int x2 = tile_x * 16 + (t & 15), y2 = tile_y * 8 + (t >> 4);
         if (x2 < geo.w[2] && y2 < geo.h[2]) {
             const int64_t o = (int64_t)y2 * geo.pitch[2] + x2;
-            put(o0, o1, geo.off[2][0] + o, 0, a.x2, b.x2);
-            put(o0, o1, geo.off[2][1] + o, 1, a.y2, b.y2);
+            emit(0, geo.off[2][0] + o, kFieldX2);
+            emit(1, geo.off[2][1] + o, kFieldY2);
         }
     }
     if (t < 32) {
@@ -469,7 +476,7 @@ VSHIP_INLINE void store_pair(const FrontJob& job, const PairGeom& geo, int tile_
         if (x3 < geo.w[3] && y3 < geo.h[3]) {
             const int64_t o = (int64_t)y3 * geo.pitch[3] + x3;
 #pragma unroll
-            for (int c = 0; c < 3; c++) put(o0, o1, geo.off[3][c] + o, c, a.l3[c], b.l3[c]);
+            for (int c = 0; c < 3; c++) emit(c, geo.off[3][c] + o, kFieldL3 + c);
         }
     }
     if (t < 8) {
@@ -477,20 +484,35 @@ VSHIP_INLINE void store_pair(const FrontJob& job, const PairGeom& geo, int tile_
         if (x4 < geo.w[4] && y4 < geo.h[4]) {
             const int64_t o = (int64_t)y4 * geo.pitch[4] + x4;
 #pragma unroll
-            for (int c = 0; c < 3; c++) put(o0, o1, geo.off[4][c] + o, c, a.l4[c], b.l4[c]);
+            for (int c = 0; c < 3; c++) emit(c, geo.off[4][c] + o, kFieldL4 + c);
         }
     }
 }
 
+// FP32 layout: each image is written as soon as it is computed. FP16 layout:
+// the reference plane holds fp16(a - offset) and the distorted plane
+// fp16(b - a) of the FP32 values, so the reference image's values wait in an
+// LDS stash (field-major, one float per thread and field) while the
+// distorted image is computed.
 template <typename StoreT>
 __global__ __launch_bounds__(kFrontThreads, 4)
 void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ counts,
                      const FrontColor* __restrict__ colors, const PairGeom* __restrict__ geoms)
 {
     extern __shared__ float lds[];
+    constexpr bool kHalf = sizeof(StoreT) == 2;
+    float* stash = lds;
+    float* work = lds + (kHalf ? kFrontStashFloats : 0);
+    const int t = threadIdx.x;
     const int total = counts[kCountFrontTiles];
     const int njobs = counts[kCountFrontJobs];
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    // workgroups go round-robin over the XCDs: each XCD takes a contiguous
+    // range of tiles, so horizontally adjacent tiles, which share the 128-byte
+    // lines of their input rows, are fetched through the same L2
+    const int per = (total + kXcds - 1) / kXcds;
+    const int xcd = blockIdx.x % kXcds;
+    const int end = min((xcd + 1) * per, total);
+    for (int item = xcd * per + blockIdx.x / kXcds; item < end; item += gridDim.x / kXcds) {
         // job lookup: largest j with tile_start <= item
         int lo = 0, hi = njobs - 1;
         while (lo < hi) {
@@ -501,10 +523,28 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
         const PairGeom& geo = geoms[job.pair];
         const int local = item - job.tile_start;
         const int tile_x = local % job.tiles_x, tile_y = local / job.tiles_x;
-        TileXyb a, b;
-        compute_tile(job, 0, colors[job.color[0]], geo, tile_x, tile_y, lds, a);
-        compute_tile(job, 1, colors[job.color[1]], geo, tile_x, tile_y, lds, b);
-        store_pair<StoreT>(job, geo, tile_x, tile_y, a, b);
+        if constexpr (!kHalf) {
+#pragma unroll 1
+            for (int k = 0; k < 2; k++) {
+                TileXyb r;
+                compute_tile(job, k, colors[job.color[k]], geo, tile_x, tile_y, work, r);
+                StoreT* out = static_cast<StoreT*>(job.out[k]);
+                store_tile(job, geo, tile_x, tile_y, [&](int, int64_t off, int f) { out[off] = field(r, f); });
+            }
+        } else {
+            TileXyb a, b;
+            compute_tile(job, 0, colors[job.color[0]], geo, tile_x, tile_y, work, a);
+#pragma unroll
+            for (int f = 0; f < kFields; f++) stash[f * kFrontThreads + t] = field(a, f);
+            compute_tile(job, 1, colors[job.color[1]], geo, tile_x, tile_y, work, b);
+            StoreT* o0 = static_cast<StoreT*>(job.out[0]);
+            StoreT* o1 = static_cast<StoreT*>(job.out[1]);
+            store_tile(job, geo, tile_x, tile_y, [&](int plane, int64_t off, int f) {
+                const float va = stash[f * kFrontThreads + t];
+                o0[off] = store_cast<StoreT>(va - kHalfOffset[plane]);
+                o1[off] = store_cast<StoreT>(field(b, f) - va);
+            });
+        }
         __syncthreads();  // LDS is reused by the next item
     }
 }
@@ -542,6 +582,8 @@ hipError_t launch_frontend(const FrontJob* jobs, const int* counts, int grid, co
                            const PairGeom* geoms, size_t lds_bytes, hipStream_t stream)
 {
     if (grid <= 0) return hipSuccess;
+    grid = (grid + kXcds - 1) / kXcds * kXcds;  // the item mapping needs whole XCD rounds
+    if (sizeof(StoreT) == 2) lds_bytes += kFrontStashFloats * sizeof(float);
     hipLaunchKernelGGL(frontend_kernel<StoreT>, dim3(grid), dim3(kFrontThreads), lds_bytes, stream,
                        jobs, counts, colors, geoms);
     return hipGetLastError();

@@ -22,9 +22,11 @@ constexpr float kC2 = 0.0009f;    // SSIM constant (score.hpp:272-273)
 
 // Front-end tile: 64x32 level-0 pixels (whole 4x2 blocks at level 4), 512
 // threads, each owning a 2x2 block.
+constexpr int kXcds = 8;  // MI355X: workgroups are dispatched round-robin over 8 XCDs, each with its own L2
 constexpr int kFrontTileW = 64;
 constexpr int kFrontTileH = 32;
 constexpr int kFrontThreads = 512;
+constexpr int kFrontStashFloats = 14 * kFrontThreads;  // FP16 path: reference image values per tile
 
 // Blur/map tiles: 256 threads, output tile shape chosen per launch (blur.hip),
 // halo 8 on every side.
@@ -69,8 +71,8 @@ struct FrontColor {
     int32_t transfer;      // Vship_TransferFunction_t
     int32_t use_pre;       // pre is not the identity
     int32_t use_post;      // post is not the identity
-    float range0[3];       // scale, offset, gain of plane 0
-    float range12[3];      // scale, offset, gain of planes 1 and 2
+    float range0[2];       // v * mul + add of plane 0
+    float range12[2];      // v * mul + add of planes 1 and 2
     float pre[9];          // row-major
     float post[9];         // row-major
 };
