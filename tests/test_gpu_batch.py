@@ -128,3 +128,47 @@ def test_batch_errors(pairs):
         with pytest.raises(S.VshipError) as e:
             b.compute(*args)
         assert e.value.code == 4
+
+
+def test_mixed_resolution_batch_with_4k_10bit_vs_oracle():
+    """One 3840x2160 10-bit pair with 1080p, 720p and 360p 8-bit pairs in one
+    batched call (the mixed-resolution workload at reduced count): FP16 storage
+    within 0.01 and FP32 storage within 0.005 of the FP64 oracle."""
+    import concurrent.futures as cf
+    from vship_amd import synth
+
+    pairs = synth.mixed_batch([1, 1, 1, 2], seed=11)
+    bufs, css, ptrs, strides = [], [], [], []
+    for w, h, bits, ref, dist, _q in pairs:
+        css.append(S.colorspace(w, h, sample=S.SAMPLE_UINT10 if bits == 10 else S.SAMPLE_UINT8))
+        strides.append(S.host_strides(ref))
+        pair = []
+        for img in (ref, dist):
+            trip = []
+            for p in img:
+                b = S.DeviceBuffer(p.nbytes)
+                b.upload(p)
+                bufs.append(b)
+                trip.append(b.address)
+            pair.append(trip)
+        ptrs.append(pair)
+    try:
+        p1, p2 = [p[0] for p in ptrs], [p[1] for p in ptrs]
+        with S.SSIMU2Batch(8, S.STORAGE_FP16) as b:
+            got16 = b.compute(css, css, p1, p2, strides, strides)
+        with S.SSIMU2Batch(8, S.STORAGE_FP32) as b:
+            got32 = b.compute(css, css, p1, p2, strides, strides)
+    finally:
+        for b in bufs:
+            b.free()
+
+    def one(i):
+        w, h, bits, ref, dist, _q = pairs[i]
+        return oracle.ssimu2(css[i], ref, strides[i], css[i], dist, strides[i])[0]
+
+    with cf.ThreadPoolExecutor(max_workers=len(pairs)) as ex:
+        want = list(ex.map(one, range(len(pairs))))
+    for i, (w, h, bits, *_rest) in enumerate(pairs):
+        _say(f"mixed {w}x{h} {bits}-bit: fp16 {got16[i]:.5f} fp32 {got32[i]:.5f} oracle {want[i]:.5f}")
+        assert abs(got16[i] - want[i]) <= 0.01
+        assert abs(got32[i] - want[i]) <= 0.005

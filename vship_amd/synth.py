@@ -147,3 +147,44 @@ def random_planes(cs, rng: np.random.Generator, pad: int = 0):
         planes.append(buf)
         strides.append(stride)
     return planes, strides
+
+
+# Mixed-resolution batch: 3840x2160 (10-bit), 1920x1080, 1280x720 and 640x360
+# (8-bit) YUV 4:2:0 BT.709 limited pairs in the ratio 1:4:8:16.
+MIXED_SIZES = ((3840, 2160, 10), (1920, 1080, 8), (1280, 720, 8), (640, 360, 8))
+MIXED_RATIO = (1, 4, 8, 16)
+
+
+def mixed_counts(total: int):
+    """Pairs per size for `total` pairs split 1:4:8:16 by largest remainders
+    (512 -> 18, 71, 141, 282)."""
+    s = sum(MIXED_RATIO)
+    exact = [total * r / s for r in MIXED_RATIO]
+    counts = [int(e) for e in exact]
+    order = sorted(range(len(exact)), key=lambda i: exact[i] - counts[i], reverse=True)
+    for i in order[: total - sum(counts)]:
+        counts[i] += 1
+    return counts
+
+
+def mixed_batch(counts, seed: int = 0, workers: int = 8):
+    """Seeded, shuffled list of (width, height, bits, ref planes, dist planes, q)
+    with counts[k] pairs of MIXED_SIZES[k]; the content and distortion are those
+    of YuvSequence."""
+    import concurrent.futures as cf
+
+    jobs = []
+    for k, ((w, h, bits), n) in enumerate(zip(MIXED_SIZES, counts)):
+        if n:
+            seq = YuvSequence(w, h, n, bits=bits, seed=seed + 101 * k)
+            jobs += [(seq, i) for i in range(n)]
+
+    def one(job):
+        seq, i = job
+        ref, dist, q = seq.frame(i)
+        return (seq.w, seq.h, seq.bits, ref, dist, q)
+
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        out = list(ex.map(one, jobs))
+    order = np.random.default_rng(seed).permutation(len(out))
+    return [out[i] for i in order]
