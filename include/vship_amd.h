@@ -109,6 +109,45 @@ EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2BatchProfile(VshipAMD_SSIMU2Batc
 /* Algorithmic HBM bytes and blur-stage FLOPs of the last batch. */
 EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2BatchWork(VshipAMD_SSIMU2Batch batch, double* bytes, double* flops);
 
+/* ---- Score surrogate (no counterpart in the reference) --------------------
+ * A 55 -> 128 -> 128 -> 2 GELU MLP with BF16 weights and FP32 accumulation
+ * that predicts the SSIMULACRA2 score and its standard deviation from the 54
+ * features of scales 0-2 (index plane * 18 + scale * 6 + k, k = ssim L1,
+ * artifact L1, detail L1, ssim L4, artifact L4, detail L4, i.e. the first
+ * 18 of each plane's 36 entries in the 108-feature vector) followed by the
+ * natural log of the pixel count. Parameters are VshipAMD_MLPParamCount()
+ * floats in the order W1[128][55], b1[128], W2[128][128], b2[128],
+ * W3[2][128], b3[2]. Sample arrays (X: n * 55 floats row-major, Y, mean,
+ * sigma: n floats) are device pointers. */
+typedef struct {
+    unsigned id;
+} VshipAMD_MLP;
+
+EXPORTPREPROCESS int VshipAMD_MLPParamCount(void);
+/* max_batch bounds the samples of one gradient/training step. */
+EXPORTPREPROCESS Vship_Exception VshipAMD_MLPInit(VshipAMD_MLP* mlp, int max_batch, int gpu_id);
+EXPORTPREPROCESS Vship_Exception VshipAMD_MLPFree(VshipAMD_MLP mlp);
+/* Sets the FP32 master weights (and their BF16 copies); resets Adam. */
+EXPORTPREPROCESS Vship_Exception VshipAMD_MLPSetParams(VshipAMD_MLP mlp, const float* params, int count);
+EXPORTPREPROCESS Vship_Exception VshipAMD_MLPGetParams(VshipAMD_MLP mlp, float* params, int count);
+/* Input columns are normalised as (x - mean) / std (std <= 0: column ignored),
+ * the target as (y - ymean) / ystd. */
+EXPORTPREPROCESS Vship_Exception VshipAMD_MLPSetNorm(VshipAMD_MLP mlp, const float* xmean, const float* xstd,
+                                                     float ymean, float ystd);
+/* Predicted score and standard deviation of n samples. */
+EXPORTPREPROCESS Vship_Exception VshipAMD_MLPPredict(VshipAMD_MLP mlp, const float* dX, int n, float* dmean,
+                                                     float* dsigma);
+/* Gradient (host, ParamCount floats) and value of the mean Gaussian negative
+ * log-likelihood over n <= max_batch samples, without an update. */
+EXPORTPREPROCESS Vship_Exception VshipAMD_MLPGradients(VshipAMD_MLP mlp, const float* dX, const float* dY, int n,
+                                                       float* grad, double* loss);
+/* Adam training over `epochs` seeded shuffles in steps of `batch` samples with
+ * a cosine learning-rate decay from lr to lr / 20; loss_per_epoch (host,
+ * epochs floats, may be NULL) receives each epoch's mean training loss. */
+EXPORTPREPROCESS Vship_Exception VshipAMD_MLPTrain(VshipAMD_MLP mlp, const float* dX, const float* dY, int n,
+                                                   int epochs, int batch, float lr, float weight_decay, uint64_t seed,
+                                                   float* loss_per_epoch);
+
 /* Device memory helpers for callers without another HIP binding. */
 EXPORTPREPROCESS Vship_Exception VshipAMD_DeviceMalloc(void** ptr, uint64_t size, int gpu_id);
 EXPORTPREPROCESS Vship_Exception VshipAMD_DeviceFree(void* ptr);

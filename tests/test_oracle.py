@@ -235,3 +235,71 @@ def test_unpruned_features_superset():
     np.testing.assert_array_equal(f1[nz], f0[nz])
     # the dropped weights sum to 0.0133, so the scores stay close
     assert abs(s1 - s0) < 0.05
+
+
+# ---- the surrogate network's FP64 restatement (oracle/mlp.py)
+
+def _mlp_case(seed=0, n=7):
+    from oracle import mlp
+    rng = np.random.default_rng(seed)
+    p = (rng.normal(0, 0.3, mlp.PARAMS)).astype(np.float32)
+    X = rng.normal(0, 2, (n, mlp.IN)).astype(np.float32)
+    Y = rng.normal(60, 15, n).astype(np.float32)
+    xmean = X.mean(0).astype(np.float32)
+    xstd = X.std(0).astype(np.float32)
+    xstd[3] = 0.0  # a constant column is ignored
+    return p, X, Y, xmean, xstd, np.float32(60.0), np.float32(15.0)
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    from oracle import mlp
+    p, X, Y, xm, xs, ym, ys = _mlp_case()
+    _, g = mlp.loss_and_grad(p, X, Y, xm, xs, ym, ys, quantise=False)
+    rng = np.random.default_rng(1)
+    # every block of the parameter vector: weights and biases of each layer
+    picks = []
+    for lo, hi in [(mlp.OFF_W1, mlp.OFF_B1), (mlp.OFF_B1, mlp.OFF_W2), (mlp.OFF_W2, mlp.OFF_B2),
+                   (mlp.OFF_B2, mlp.OFF_W3), (mlp.OFF_W3, mlp.OFF_B3), (mlp.OFF_B3, mlp.PARAMS)]:
+        picks += list(rng.integers(lo, hi, 6))
+    pd = p.astype(np.float64)
+    for k in picks:
+        h = 1e-3
+        up, dn = pd.copy(), pd.copy()
+        up[k] += h
+        dn[k] -= h
+        # float64 parameters through the unquantised path
+        lu, _ = mlp.loss_and_grad(up, X, Y, xm, xs, ym, ys, quantise=False)
+        ld, _ = mlp.loss_and_grad(dn, X, Y, xm, xs, ym, ys, quantise=False)
+        fd = (lu - ld) / (2 * h)
+        assert fd == pytest.approx(g[k], rel=2e-3, abs=1e-6), k
+
+
+def test_mlp_oracle_gelu_and_loss_closed_forms():
+    from math import erf, exp, pi, sqrt
+    from oracle import mlp
+    z = np.array([-3.0, -1.0, 0.0, 0.5, 2.0])
+    want = [0.5 * v * (1 + erf(v / sqrt(2))) for v in z]
+    np.testing.assert_allclose(mlp.gelu(z), want, rtol=1e-15)
+    # derivative of x * Phi(x) is Phi(x) + x * phi(x); at 0 it is 1/2
+    assert mlp.gelu_grad(np.array([0.0]))[0] == 0.5
+    dz = 1e-6
+    for v in z:
+        fd = (mlp.gelu(np.array([v + dz])) - mlp.gelu(np.array([v - dz])))[0] / (2 * dz)
+        assert mlp.gelu_grad(np.array([v]))[0] == pytest.approx(fd, rel=1e-7)
+    assert mlp.gelu_grad(np.array([1.0]))[0] == pytest.approx(0.5 * (1 + erf(1 / sqrt(2))) + exp(-0.5) / sqrt(2 * pi))
+    # all-zero weights: mu = lv = b3 = 0, so the loss is the mean of y^2 / 2
+    p = np.zeros(mlp.PARAMS, np.float32)
+    X = np.ones((4, mlp.IN), np.float32)
+    Y = np.array([1.0, -2.0, 0.5, 3.0], np.float32)
+    loss, g = mlp.loss_and_grad(p, X, Y, np.zeros(mlp.IN, np.float32), np.ones(mlp.IN, np.float32), 0.0, 1.0)
+    assert loss == pytest.approx(np.mean(Y.astype(np.float64) ** 2) / 2)
+    # d loss / d b3 = (-mean(y), mean(0.5 * (1 - y^2)))
+    assert g[mlp.OFF_B3] == pytest.approx(-np.mean(Y))
+    assert g[mlp.OFF_B3 + 1] == pytest.approx(np.mean(0.5 * (1 - Y.astype(np.float64) ** 2)))
+
+
+def test_bf16_rounding_ties_to_even():
+    from oracle import mlp
+    # 1 + 2^-8 is halfway between bf16 1 and 1 + 2^-7: ties go to the even 1.0
+    vals = np.array([1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -9, -2.5], np.float32)
+    np.testing.assert_array_equal(mlp.bf16_round(vals), np.array([1.0, 1.0 + 2 ** -6, 1.0, -2.5], np.float32))

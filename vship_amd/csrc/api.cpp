@@ -22,6 +22,7 @@
 #include "engine.hpp"
 #include "errors.hpp"
 #include "kernels.hpp"
+#include "mlp.hpp"
 
 namespace vship {
 
@@ -67,6 +68,7 @@ struct HandlerSlot {
 std::mutex g_lock;
 std::vector<HandlerSlot> g_handlers;
 std::vector<std::unique_ptr<Ssimu2Batch>> g_batches;
+std::vector<std::unique_ptr<Surrogate>> g_mlps;
 std::string g_last_error = Error(Vship_NoError, __FILE__, __LINE__).message();
 int g_device = 0;
 
@@ -153,6 +155,13 @@ Ssimu2Batch* lookup_batch(unsigned id)
     if (id >= g_batches.size() || !g_batches[id])
         throw Error(Vship_BadHandler, __FILE__, __LINE__, "invalid batch id");
     return g_batches[id].get();
+}
+
+Surrogate* lookup_mlp(unsigned id)
+{
+    std::lock_guard<std::mutex> g(g_lock);
+    if (id >= g_mlps.size() || !g_mlps[id]) throw Error(Vship_BadHandler, __FILE__, __LINE__, "invalid surrogate id");
+    return g_mlps[id].get();
 }
 
 BatchInput make_batch_input(int npairs, const Vship_Colorspace_t* src, const Vship_Colorspace_t* dis,
@@ -574,6 +583,88 @@ Vship_Exception VshipAMD_SSIMU2BatchWork(VshipAMD_SSIMU2Batch batch, double* byt
 {
     if (!bytes || !flops) return Vship_BadPointer;
     return guarded([&] { lookup_batch(batch.id)->work(bytes, flops); });
+}
+
+int VshipAMD_MLPParamCount(void) { return kMlpParams; }
+
+Vship_Exception VshipAMD_MLPInit(VshipAMD_MLP* mlp, int max_batch, int gpu_id)
+{
+    if (!mlp) return Vship_BadPointer;
+    return guarded([&] {
+        const int n = device_count();
+        if (gpu_id < 0 || gpu_id >= n) VSHIP_THROW(Vship_BadDeviceArgument, "In VshipAMD_MLPInit");
+        std::unique_ptr<Surrogate> m(new Surrogate(max_batch, gpu_id));
+        std::lock_guard<std::mutex> g(g_lock);
+        unsigned id = 0;
+        while (id < g_mlps.size() && g_mlps[id]) id++;
+        if (id == g_mlps.size()) g_mlps.emplace_back();
+        g_mlps[id] = std::move(m);
+        mlp->id = id;
+    });
+}
+
+Vship_Exception VshipAMD_MLPFree(VshipAMD_MLP mlp)
+{
+    std::unique_ptr<Surrogate> m;
+    {
+        std::lock_guard<std::mutex> g(g_lock);
+        if (mlp.id >= g_mlps.size() || !g_mlps[mlp.id]) {
+            g_last_error = Error(Vship_BadHandler, __FILE__, __LINE__, "invalid surrogate id in VshipAMD_MLPFree").message();
+            return Vship_BadHandler;
+        }
+        m = std::move(g_mlps[mlp.id]);
+    }
+    return guarded([&] { m.reset(); });
+}
+
+Vship_Exception VshipAMD_MLPSetParams(VshipAMD_MLP mlp, const float* params, int count)
+{
+    return guarded([&] {
+        if (!params) VSHIP_THROW(Vship_BadPointer, "null parameter array");
+        lookup_mlp(mlp.id)->set_params(params, count);
+    });
+}
+
+Vship_Exception VshipAMD_MLPGetParams(VshipAMD_MLP mlp, float* params, int count)
+{
+    return guarded([&] {
+        if (!params) VSHIP_THROW(Vship_BadPointer, "null parameter array");
+        lookup_mlp(mlp.id)->get_params(params, count);
+    });
+}
+
+Vship_Exception VshipAMD_MLPSetNorm(VshipAMD_MLP mlp, const float* xmean, const float* xstd, float ymean, float ystd)
+{
+    return guarded([&] {
+        if (!xmean || !xstd) VSHIP_THROW(Vship_BadPointer, "null normalisation array");
+        lookup_mlp(mlp.id)->set_norm(xmean, xstd, ymean, ystd);
+    });
+}
+
+Vship_Exception VshipAMD_MLPPredict(VshipAMD_MLP mlp, const float* dX, int n, float* dmean, float* dsigma)
+{
+    return guarded([&] {
+        if (n > 0 && (!dX || !dmean || !dsigma)) VSHIP_THROW(Vship_BadPointer, "null sample array");
+        lookup_mlp(mlp.id)->predict(dX, n, dmean, dsigma);
+    });
+}
+
+Vship_Exception VshipAMD_MLPGradients(VshipAMD_MLP mlp, const float* dX, const float* dY, int n, float* grad,
+                                      double* loss)
+{
+    return guarded([&] {
+        if (!dX || !dY || !grad) VSHIP_THROW(Vship_BadPointer, "null argument");
+        lookup_mlp(mlp.id)->gradients(dX, dY, n, grad, loss);
+    });
+}
+
+Vship_Exception VshipAMD_MLPTrain(VshipAMD_MLP mlp, const float* dX, const float* dY, int n, int epochs, int batch,
+                                  float lr, float weight_decay, uint64_t seed, float* loss_per_epoch)
+{
+    return guarded([&] {
+        if (!dX || !dY) VSHIP_THROW(Vship_BadPointer, "null sample array");
+        lookup_mlp(mlp.id)->train(dX, dY, n, epochs, batch, lr, weight_decay, seed, loss_per_epoch);
+    });
 }
 
 Vship_Exception VshipAMD_DeviceMalloc(void** ptr, uint64_t size, int gpu_id)
