@@ -12,7 +12,8 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
     lib = os.path.join(ROOT, "vship_amd", "libvship_amd.so")
     olib = os.path.join(ROOT, "oracle", "liboracle.so")
-    if not (os.path.exists(lib) and os.path.exists(olib)):
+    mlib = os.path.join(ROOT, "tests", "vs_host", "libmock_vs.so")
+    if not (os.path.exists(lib) and os.path.exists(olib) and os.path.exists(mlib)):
         import __graft_entry__
         __graft_entry__.build()
 
