@@ -420,6 +420,7 @@ static const VSFrame* evaluate(VSNode* node, int n, std::string& err)
     const VSFrame* out = node->get(n, arInitial, node->instance, &frame_data, &ctx, &g_core, &g_api);
     if (!out && ctx.error.empty()) {
         for (auto& r : ctx.requested) {
+            if (ctx.ready.count(r)) continue;  // the same frame requested twice (e.g. one clip as both inputs)
             std::string e2;
             const VSFrame* f = evaluate(r.first, r.second, e2);
             if (!f) {

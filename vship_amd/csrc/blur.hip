@@ -28,14 +28,6 @@ namespace vship {
 namespace {
 
 
-__device__ inline float inv_norm(int x, int w, const BlurTaps& taps)
-{
-    if (x >= w) return 0.0f;
-    const int beg = x < 8 ? 8 - x : 0;
-    const int end = (w - x + 8) < 17 ? (w - x + 8) : 17;
-    return 1.0f / (taps.integral[end] - taps.integral[beg]);
-}
-
 template <typename StoreT>
 __device__ inline void load4(const StoreT* p, float v[4]);
 
@@ -302,11 +294,12 @@ BlurTile blur_tile_shape(int config)
     case 2: return {32, 64};
     case 3: return {32, 32};
     case 4: return {64, 64};
+    case 5: return {64, 48};
     default: return {64, 32};
     }
 }
 
-int blur_tile_configs() { return 5; }
+int blur_tile_configs() { return 6; }
 
 int blur_grid_for(int total_tiles)
 {
@@ -324,6 +317,7 @@ hipError_t launch_blur(int config, const MapJob* jobs, const int* counts, int gr
     case 2: return launch_one<StoreT, 32, 64>(jobs, counts, grid, taps, partials, stream);
     case 3: return launch_one<StoreT, 32, 32>(jobs, counts, grid, taps, partials, stream);
     case 4: return launch_one<StoreT, 64, 64>(jobs, counts, grid, taps, partials, stream);
+    case 5: return launch_blur_mfma<StoreT>(jobs, counts, grid, taps, partials, stream);
     default: return launch_one<StoreT, 64, 32>(jobs, counts, grid, taps, partials, stream);
     }
 }

@@ -7,6 +7,31 @@
 
 namespace vship {
 
+// 1 / (sum of the in-bounds taps) at position x of an axis of length w
+// (reference gaussianblur.hpp:52-54,68-70: integral[end] - integral[beg]);
+// 0 outside the axis. Away from the borders all 17 taps are in bounds and
+// the value is the precomputed taps.inv_full (the same float expression).
+__device__ inline float inv_norm(int x, int w, const BlurTaps& taps)
+{
+    if (x >= kHalo && x + kHalo < w) return taps.inv_full;
+    if (x >= w) return 0.0f;
+    const int beg = x < kHalo ? kHalo - x : 0;
+    const int end = (w - x + kHalo) < 17 ? (w - x + kHalo) : 17;
+    return 1.0f / (taps.integral[end] - taps.integral[beg]);
+}
+
+// n / d from the hardware reciprocal and one Newton correction: within an
+// ulp of IEEE division, and exactly 1 when n == d (the residual n - d * q is
+// exact and the correction lands on 1), which keeps identical inputs at a
+// score of exactly 100.
+__device__ inline float div_nr(float n, float d)
+{
+    const float r = __builtin_amdgcn_rcpf(d);
+    const float q = n * r;
+    const float e = fmaf(-d, q, n);
+    return fmaf(e, r, q);
+}
+
 struct MapSums {
     float s1[3];  // sum of ssim, artifact, detail-loss
     float s4[3];  // sum of their 4th powers
@@ -29,7 +54,7 @@ __device__ inline float ssim_value(float m1, float m2, float s12, float sq)
     const float num_m = 1.0f - md * md;
     const float num_s = 2.0f * (s12 - m12) + kC2;
     const float den_s = (sq - (m11 + m22)) + kC2;
-    return fmaxf(1.0f - (num_m * num_s) / den_s, 0.0f);
+    return fmaxf(1.0f - div_nr(num_m * num_s, den_s), 0.0f);
 }
 
 // Edge difference d = (|b - mu2| + 1) / (|a - mu1| + 1) - 1, artifact =
@@ -38,7 +63,7 @@ __device__ inline float ssim_value(float m1, float m2, float s12, float sq)
 __device__ inline float edge_value(float a, float b, float m1, float m2)
 {
 #pragma clang fp contract(off)
-    return (fabsf(b - m2) + 1.0f) / (fabsf(a - m1) + 1.0f) - 1.0f;
+    return div_nr(fabsf(b - m2) + 1.0f, fabsf(a - m1) + 1.0f) - 1.0f;
 }
 
 __device__ inline float pow4(float x)

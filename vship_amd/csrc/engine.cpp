@@ -250,6 +250,8 @@ BlurTaps make_blur_taps()
         t.g[i] = (float)((double)e / std::sqrt(tau * sigma * sigma));
         t.integral[i + 1] = t.integral[i] + t.g[i];
     }
+    // the same float expression as the kernels' border case with beg = 0, end = 17
+    t.inv_full = 1.0f / (t.integral[17] - t.integral[0]);
     return t;
 }
 

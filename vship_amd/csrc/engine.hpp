@@ -43,7 +43,7 @@ struct PairLayout {
     int blur_config;
 };
 
-PairLayout make_pair_layout(int w, int h, int blur_config = 0);
+PairLayout make_pair_layout(int w, int h, int blur_config = kDefaultBlurConfig);
 
 class Ssimu2Handler {
 public:

@@ -17,7 +17,8 @@ hipError_t launch_frontend(const FrontJob* jobs, const int* counts, int grid, co
                            const PairGeom* geoms, size_t lds_bytes, hipStream_t stream);
 
 // Blur/map tile shapes: config 0 = 64x32 (default), 1 = 64x16, 2 = 32x64,
-// 3 = 32x32, 4 = 64x64.
+// 3 = 32x32, 4 = 64x64 (FP32 VALU kernel); 5 = 64x48 (matrix-core kernel,
+// blur_mfma.hip).
 struct BlurTile {
     int tw, th;
 };
@@ -31,6 +32,11 @@ int blur_grid_for(int total_tiles);
 template <typename StoreT>
 hipError_t launch_blur(int config, const MapJob* jobs, const int* counts, int grid, const BlurTaps& taps,
                        float* partials, hipStream_t stream);
+
+// Matrix-core variant of launch_blur (tile 64x48, blur config 5).
+template <typename StoreT>
+hipError_t launch_blur_mfma(const MapJob* jobs, const int* counts, int grid, const BlurTaps& taps, float* partials,
+                            hipStream_t stream);
 
 hipError_t launch_score(const ScoreJob* pairs, const int* counts, int grid, const MapJob* jobs,
                         const float* partials, double* features, double* scores, hipStream_t stream);

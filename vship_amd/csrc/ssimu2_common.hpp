@@ -23,6 +23,7 @@ constexpr float kC2 = 0.0009f;    // SSIM constant (score.hpp:272-273)
 // Front-end tile: 64x32 level-0 pixels (whole 4x2 blocks at level 4), 512
 // threads, each owning a 2x2 block.
 constexpr int kXcds = 8;  // MI355X: workgroups are dispatched round-robin over 8 XCDs, each with its own L2
+constexpr int kDefaultBlurConfig = 5;  // blur tile config: 5 = the matrix-core kernel, fastest measured on MI355X
 constexpr int kFrontTileW = 64;
 constexpr int kFrontTileH = 32;
 constexpr int kFrontThreads = 512;
@@ -133,6 +134,7 @@ struct MapJob {
 struct BlurTaps {
     float g[17];
     float integral[18];
+    float inv_full;  // 1 / (integral[17] - integral[0]): all 17 taps in bounds
 };
 
 // Per-pair score job for the reduction kernel.

@@ -2,6 +2,7 @@
 
 python -m vship_amd.profile_workload plain --pairs 8     # plain FP32 path, 1080p YUV420
 python -m vship_amd.profile_workload batch --pairs 240   # batched FP16 path, 1080p YUV420
+python -m vship_amd.profile_workload batch --blur-config 5  # same, matrix-core blur
 python -m vship_amd.profile_workload copy --mib 256      # plain copy kernel (counter calibration)
 
 Each mode prints what it ran and the HIP-event time per kernel.
@@ -45,6 +46,7 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--mib", type=int, default=256)
     ap.add_argument("--storage", choices=["fp16", "fp32"], default="fp16")
+    ap.add_argument("--blur-config", type=int, default=None, help="batch: blur tile config (5 = matrix cores)")
     a = ap.parse_args(argv)
     if a.mode == "copy":
         nbytes = a.mib << 20
@@ -67,6 +69,9 @@ def main(argv=None):
     else:
         storage = S.STORAGE_FP16 if a.storage == "fp16" else S.STORAGE_FP32
         with S.SSIMU2Batch(a.pairs, storage) as b:
+            if a.blur_config is not None:
+                t = b.tuning()
+                b.set_tuning(a.blur_config, t["blur_waves"], t["front_waves"])
             args = ([cs] * a.pairs, [cs] * a.pairs, [p[0] for p in ptrs], [p[1] for p in ptrs], [st] * a.pairs,
                     [st] * a.pairs)
             for _ in range(a.reps):
