@@ -222,7 +222,7 @@ class SurrogateNet:
 # ---------------------------------------------------------------- training data
 
 SIZES = ((256, 256), (384, 216), (512, 288), (640, 360), (960, 540), (1280, 720), (1920, 1080))
-SIZE_PROB = (0.35, 0.20, 0.15, 0.12, 0.10, 0.05, 0.03)
+SIZE_PROB = (0.15, 0.10, 0.10, 0.15, 0.15, 0.15, 0.20)
 _D8 = synth._DCT8
 
 
@@ -273,13 +273,15 @@ def distort(planes, q: float, blur: float, noise: float, rng: np.random.Generato
 
 
 class TrainingPairs:
-    """Seeded source pool and per-pair distortion draws (see module doc)."""
+    """Seeded source pool (sources_per_size independent frames per size) and
+    per-pair distortion draws (see module doc)."""
 
-    def __init__(self, seed: int = 0, sources_per_size: int = 24, sizes=SIZES, size_prob=SIZE_PROB):
+    def __init__(self, seed: int = 0, sources_per_size: int = 16, sizes=SIZES, size_prob=SIZE_PROB):
         self.seed = seed
         self.sizes = sizes
         self.prob = np.asarray(size_prob, dtype=np.float64) / np.sum(size_prob)
-        self.pool = [synth.YuvSequence(w, h, sources_per_size, seed=seed * 1000 + k, shift=5)
+        # every source is its own random sinusoid-sum / chroma-field draw
+        self.pool = [[synth.YuvSequence(w, h, 1, seed=(seed * 1000 + k) * 1000 + j) for j in range(sources_per_size)]
                      for k, (w, h) in enumerate(sizes)]
         self.sources_per_size = sources_per_size
         nrng = np.random.default_rng([seed, 7])
@@ -290,9 +292,8 @@ class TrainingPairs:
         """(width, height, ref planes, dist planes, (q, blur, noise)) of pair i."""
         rng = np.random.default_rng([self.seed, i])
         k = int(rng.choice(len(self.sizes), p=self.prob))
-        seq = self.pool[k]
-        j = int(rng.integers(0, self.sources_per_size))
-        ref = self.source(seq, j)
+        seq = self.pool[k][int(rng.integers(0, self.sources_per_size))]
+        ref = self.source(seq, 0)
         q = float(np.exp(rng.uniform(np.log(2.0), np.log(64.0))))
         # a third of the pairs without blur and a third without noise, so pure
         # quantisation (the usual encoder candidate) is well covered
