@@ -120,22 +120,40 @@ __device__ void blur_tile_mfma(const MapJob& job, int tile_x, int tile_y, const 
     // padded to 64 elements, so every vector load stays inside its row)
     const float c = load_elem<StoreT>(A + (int64_t)ty0 * pitch + tx0);
     float vmax = 0.0f;
-    for (int e = t; e < kWR * (kWC / 4); e += kBlurThreads) {
+    // every thread stages exactly kStageIters items; all loads are issued
+    // before any is used
+    constexpr int kItems = kWR * (kWC / 4);
+    constexpr int kStageIters = kItems / kBlurThreads;
+    static_assert(kItems % kBlurThreads == 0, "staging items per thread");
+    float ra[kStageIters][4], rb[kStageIters][4];
+    bool ok[kStageIters];
+#pragma unroll
+    for (int it = 0; it < kStageIters; it++) {
+        const int e = t + it * kBlurThreads;
         const int v = e / (kWC / 4);
         const int u = 4 * (e - v * (kWC / 4));
         const int gy = ty0 - kHalo + v, gx = tx0 - kHalo + u;
-        float va[4] = {0.f, 0.f, 0.f, 0.f}, vb[4] = {0.f, 0.f, 0.f, 0.f};
-        if (gy >= 0 && gy < h && gx >= 0 && gx < pitch && u < kTW + 2 * kHalo) {
+        ok[it] = gy >= 0 && gy < h && gx >= 0 && gx < pitch && u < kTW + 2 * kHalo;
+        if (ok[it]) {
             const int64_t o = (int64_t)gy * pitch + gx;
-            float ra[4], rb[4];
-            load4v<StoreT>(A + o, ra);
-            load4v<StoreT>(B + o, rb);
+            load4v<StoreT>(A + o, ra[it]);
+            load4v<StoreT>(B + o, rb[it]);
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < kStageIters; it++) {
+        const int e = t + it * kBlurThreads;
+        const int v = e / (kWC / 4);
+        const int u = 4 * (e - v * (kWC / 4));
+        const int gx = tx0 - kHalo + u;
+        float va[4] = {0.f, 0.f, 0.f, 0.f}, vb[4] = {0.f, 0.f, 0.f, 0.f};
+        if (ok[it]) {
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 // FP16 planes hold the distorted image as its difference from
                 // the reference: b = a + d
-                const float bv = (sizeof(StoreT) == 2) ? ra[i] + rb[i] : rb[i];
-                va[i] = (gx + i < w) ? ra[i] - c : 0.f;
+                const float bv = (sizeof(StoreT) == 2) ? ra[it][i] + rb[it][i] : rb[it][i];
+                va[i] = (gx + i < w) ? ra[it][i] - c : 0.f;
                 vb[i] = (gx + i < w) ? bv - c : 0.f;
             }
         }
