@@ -209,3 +209,66 @@ def test_against_reference_libvship():
             assert abs(got - want_ref) <= abs(want_ref - want_or) + SCORE_TOL, name
         else:
             assert abs(got - want_ref) <= SCORE_TOL, name
+
+
+RESAMPLE_CASES = [
+    # (name, ref (w, h, target, crop), dist (w, h, target, crop)); crop = (top, bottom, left, right)
+    ("upscale_yuv420", (320, 180, (640, 360), (0, 0, 0, 0)), (320, 180, (640, 360), (0, 0, 0, 0))),
+    ("downscale_and_crop", (640, 360, (256, 150), (4, 2, 6, 3)), (640, 360, (256, 150), (4, 2, 6, 3))),
+    ("crop_only", (200, 120, (-1, -1), (3, 5, 7, 2)), (200, 120, (-1, -1), (3, 5, 7, 2))),
+    ("different_sources_same_final", (320, 180, (480, 270), (0, 0, 0, 0)), (480, 270, (-1, -1), (0, 0, 0, 0))),
+]
+
+
+@pytest.mark.parametrize("name,rspec,dspec", RESAMPLE_CASES, ids=[c[0] for c in RESAMPLE_CASES])
+def test_crop_and_resize_vs_oracle(name, rspec, dspec):
+    """target_width/height (cubic-Hermite resize in linear light) and crop,
+    reference gpuColorToLinear/vshipColor.hpp:87-103, against the FP64 oracle;
+    the pair's two images may come from different sizes."""
+    def make(spec, seed):
+        w, h, (tw, th), crop = spec
+        cs = S.colorspace(w, h)
+        cs.target_width, cs.target_height = tw, th
+        cs.crop = S.CropRectangle(*crop)
+        seq = synth.YuvSequence(w, h, 2, seed=seed)
+        return cs, seq
+    cs1, seq1 = make(rspec, 5)
+    cs2, seq2 = make(dspec, 5)
+    ref, _, _ = seq1.frame(0)
+    if rspec[:2] == dspec[:2]:
+        _, dist, _ = seq1.frame(0, q=9.0)
+    else:
+        # same content at the distorted image's own size, then quantised
+        _, dist, _ = seq2.frame(0, q=9.0)
+    s1, s2 = S.host_strides(ref), S.host_strides(dist)
+    with S.SSIMU2Handler(cs1, cs2) as h:
+        got = h.compute(ref, dist, s1, s2)
+        ident = h.compute(ref, ref, s1, s1) if rspec == dspec else None
+    want, _ = oracle.ssimu2(cs1, ref, s1, cs2, dist, s2)
+    _say(f"resample {name}: gpu {got:.6f} oracle {want:.6f} delta {got - want:+.2e}")
+    assert abs(got - want) <= SCORE_TOL
+    if ident is not None:
+        assert ident == 100.0
+
+
+def test_crop_and_resize_errors():
+    cs = S.colorspace(64, 48)
+    bad = S.colorspace(64, 48)
+    bad.crop = S.CropRectangle(30, 18, 0, 0)  # leaves no row
+    with pytest.raises(S.VshipError) as e:
+        S.SSIMU2Handler(bad, bad)
+    assert e.value.code == 5
+    other = S.colorspace(64, 48)
+    other.target_width = 80
+    with pytest.raises(S.VshipError) as e:
+        S.SSIMU2Handler(cs, other)  # final sizes differ
+    assert e.value.code == 4
+    with S.SSIMU2Batch(2, S.STORAGE_FP16) as b:
+        d = S.DeviceBuffer(64 * 48 * 2)
+        try:
+            ptrs = [d.address, d.address + 64 * 48, d.address + 64 * 48 + 32 * 24]
+            with pytest.raises(S.VshipError) as e:
+                b.compute([other], [other], [ptrs], [ptrs], [[64, 32, 32]], [[64, 32, 32]])
+            assert e.value.code == 5
+        finally:
+            d.free()

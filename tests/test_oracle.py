@@ -303,3 +303,51 @@ def test_bf16_rounding_ties_to_even():
     # 1 + 2^-8 is halfway between bf16 1 and 1 + 2^-7: ties go to the even 1.0
     vals = np.array([1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -9, -2.5], np.float32)
     np.testing.assert_array_equal(mlp.bf16_round(vals), np.array([1.0, 1.0 + 2 ** -6, 1.0, -2.5], np.float32))
+
+
+# ---- crop and resize (reference gpuColorToLinear/resize.hpp:26-65, vshipColor.hpp:87-103)
+
+def _linear_cs(w, h, tw=-1, th=-1, crop=(0, 0, 0, 0)):
+    cs = S.colorspace(w, h, sample=S.SAMPLE_FLOAT, rng=S.RANGE_FULL, subw=0, subh=0, family=S.FAMILY_RGB,
+                      matrix=S.MATRIX_RGB, trc=S.TRC_LINEAR, prim=S.PRIM_BT709)
+    cs.target_width, cs.target_height = tw, th
+    cs.crop = S.CropRectangle(*crop)  # top, bottom, left, right
+    return cs
+
+
+def _linear_planes(img):
+    planes = [np.ascontiguousarray(p, dtype=np.float32) for p in img]
+    return planes, [p.strides[0] for p in planes]
+
+
+def test_resize_constant_and_integer_downscale():
+    rng = np.random.default_rng(3)
+    const = [np.full((30, 44), v, np.float32) for v in (0.2, 0.5, 0.9)]
+    out = oracle.linear_rgb(_linear_cs(44, 30, 71, 13), *_linear_planes(const))
+    np.testing.assert_allclose(out, np.stack([np.full((13, 71), v) for v in (0.2, 0.5, 0.9)]), rtol=1e-6)
+    # halving: pos = 2x exactly, t = 0, so the output is every other sample
+    img = [rng.uniform(0, 1, (40, 60)).astype(np.float32) for _ in range(3)]
+    out = oracle.linear_rgb(_linear_cs(60, 40, 30, 20), *_linear_planes(img))
+    np.testing.assert_allclose(out, np.stack([p[::2, ::2] for p in img]).astype(np.float64), rtol=0, atol=0)
+
+
+def test_resize_reproduces_a_linear_ramp_in_the_interior():
+    # Catmull-Rom reproduces linear functions where all four taps are inside
+    w, h, tw = 20, 6, 50
+    ramp = np.tile(0.1 + 0.03 * np.arange(w, dtype=np.float32), (h, 1))
+    out = oracle.linear_rgb(_linear_cs(w, h, tw, h), *_linear_planes([ramp] * 3))
+    pos = np.arange(tw) * w / tw
+    inner = (pos >= 1) & (np.floor(pos) + 2 <= w - 1)
+    np.testing.assert_allclose(out[0][:, inner], np.tile(0.1 + 0.03 * pos[inner], (h, 1)), rtol=1e-6)
+
+
+def test_crop_is_a_window_of_the_resized_image():
+    rng = np.random.default_rng(4)
+    img = [rng.uniform(0, 1, (24, 32)).astype(np.float32) for _ in range(3)]
+    full = oracle.linear_rgb(_linear_cs(32, 24, 48, 36), *_linear_planes(img))
+    crop = oracle.linear_rgb(_linear_cs(32, 24, 48, 36, crop=(3, 5, 7, 2)), *_linear_planes(img))
+    assert crop.shape == (3, 36 - 3 - 5, 48 - 7 - 2)
+    np.testing.assert_array_equal(crop, full[:, 3:36 - 5, 7:48 - 2])
+    # crop without resize is a plain window
+    c2 = oracle.linear_rgb(_linear_cs(32, 24, crop=(1, 2, 3, 4)), *_linear_planes(img))
+    np.testing.assert_array_equal(c2, np.stack(img)[:, 1:22, 3:28].astype(np.float64))

@@ -89,6 +89,8 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
     for (int i = 0; i < in.npairs; i++) {
         const ImageSpec s1 = make_image_spec(in.src[i]);
         const ImageSpec s2 = make_image_spec(in.dis[i]);
+        if (s1.resample || s2.resample)
+            VSHIP_THROW(Vship_NonRGBSInput, "crop and resize are supported by the single-pair handler only");
         if (s1.width != s2.width || s1.height != s2.height)
             VSHIP_THROW(Vship_DifferingInputType, "pair " + std::to_string(i) + ": sizes differ");
         const ImageSpec* specs[2] = {&s1, &s2};

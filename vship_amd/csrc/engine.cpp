@@ -194,11 +194,23 @@ ImageSpec make_image_spec(const Vship_Colorspace_t& cs)
     case Vship_PRIMARIES_BT2020: break;
     default: VSHIP_THROW(Vship_NonRGBSInput, "unknown Vship_Primaries_t value");
     }
+    // resize then crop, reference vshipColor.hpp:46-61,87-103
+    if ((cs.target_width != -1 && (cs.target_width < 1 || cs.target_width > (1 << 20))) ||
+        (cs.target_height != -1 && (cs.target_height < 1 || cs.target_height > (1 << 20))))
+        VSHIP_THROW(Vship_NonRGBSInput, "target_width and target_height must be -1 or in [1, 2^20]");
     const int64_t tw = cs.target_width == -1 ? cs.width : cs.target_width;
     const int64_t th = cs.target_height == -1 ? cs.height : cs.target_height;
+    if (cs.crop.top < 0 || cs.crop.bottom < 0 || cs.crop.left < 0 || cs.crop.right < 0 ||
+        cs.crop.left + cs.crop.right >= tw || cs.crop.top + cs.crop.bottom >= th)
+        VSHIP_THROW(Vship_NonRGBSInput, "crop must be non-negative and leave at least one pixel");
     const bool cropped = cs.crop.top || cs.crop.bottom || cs.crop.left || cs.crop.right;
-    if (cropped || tw != cs.width || th != cs.height)
-        VSHIP_THROW(Vship_NonRGBSInput, "crop and resize are not supported by this library");
+    s.resample = cropped || tw != cs.width || th != cs.height;
+    s.target_w = tw;
+    s.target_h = th;
+    s.crop_left = cs.crop.left;
+    s.crop_top = cs.crop.top;
+    s.final_w = tw - cs.crop.left - cs.crop.right;
+    s.final_h = th - cs.crop.top - cs.crop.bottom;
 
     FrontColor& fc = s.fc;
     const int bits = sample_bits(cs.sample);
@@ -237,6 +249,17 @@ ImageSpec make_image_spec(const Vship_Colorspace_t& cs)
         s.plane_h[p] = ((cs.height - 1) >> fc.subh) + 1;
     }
     return s;
+}
+
+FrontColor linear_rgb_color()
+{
+    FrontColor fc;
+    std::memset(&fc, 0, sizeof(fc));
+    fc.sample = kSampleFloat;
+    fc.transfer = Vship_TRC_Linear;
+    fc.range0[0] = fc.range12[0] = 1.0f;  // v * 1 + 0
+    fc.range0[1] = fc.range12[1] = 0.0f;
+    return fc;
 }
 
 BlurTaps make_blur_taps()
@@ -316,13 +339,15 @@ Ssimu2Handler::Ssimu2Handler(const Vship_Colorspace_t& src, const Vship_Colorspa
 {
     spec_[0] = make_image_spec(src);
     spec_[1] = make_image_spec(dis);
-    if (spec_[0].width != spec_[1].width || spec_[0].height != spec_[1].height)
+    if (spec_[0].final_w != spec_[1].final_w || spec_[0].final_h != spec_[1].final_h)
         VSHIP_THROW(Vship_DifferingInputType, "reference and distorted sizes differ");
-    width_ = (int)spec_[0].width;
-    height_ = (int)spec_[0].height;
+    width_ = (int)spec_[0].final_w;
+    height_ = (int)spec_[0].final_h;
     layout_ = make_pair_layout(width_, height_);
     taps_ = make_blur_taps();
-    front_lds_ = std::max(frontend_lds_bytes(spec_[0].fc), frontend_lds_bytes(spec_[1].fc));
+    const FrontColor lin_fc = linear_rgb_color();
+    front_lds_ = std::max(std::max(frontend_lds_bytes(spec_[0].fc), frontend_lds_bytes(spec_[1].fc)),
+                          frontend_lds_bytes(lin_fc));
 
     VSHIP_HIP_CHECK(hipSetDevice(device_));
     VSHIP_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
@@ -332,7 +357,7 @@ Ssimu2Handler::Ssimu2Handler(const Vship_Colorspace_t& src, const Vship_Colorspa
     const size_t elems = (size_t)layout_.geom.image_elems;
     for (int i = 0; i < 2; i++) dmalloc(&xyb_[i], elems * sizeof(float));
     dmalloc((void**)&d_front_jobs_, sizeof(FrontJob));
-    dmalloc((void**)&d_colors_, 2 * sizeof(FrontColor));
+    dmalloc((void**)&d_colors_, 3 * sizeof(FrontColor));
     dmalloc((void**)&d_geom_, sizeof(PairGeom));
     dmalloc((void**)&d_maps_, layout_.maps.size() * sizeof(MapJob));
     dmalloc((void**)&d_score_jobs_, sizeof(ScoreJob));
@@ -351,7 +376,7 @@ Ssimu2Handler::Ssimu2Handler(const Vship_Colorspace_t& src, const Vship_Colorspa
         m.a = static_cast<const float*>(xyb_[0]) + off;
         m.b = static_cast<const float*>(xyb_[1]) + off;
     }
-    FrontColor colors[2] = {spec_[0].fc, spec_[1].fc};
+    FrontColor colors[3] = {spec_[0].fc, spec_[1].fc, lin_fc};
     ScoreJob sj = {0, (int32_t)maps.size(), {0, 0}};
     int counts[kCountSlots] = {};
     counts[kCountFrontTiles] = layout_.front_tiles_per_image;
@@ -376,6 +401,40 @@ Ssimu2Handler::Ssimu2Handler(const Vship_Colorspace_t& src, const Vship_Colorspa
     j.tiles_x = layout_.front_tiles_x;
     j.tile_start = 0;
     j.pair = 0;
+
+    // crop/resize path: linear RGB at the input size, resized, then the
+    // pyramid kernel reads the cropped window as float linear RGB
+    if (spec_[0].resample || spec_[1].resample) {
+        dmalloc((void**)&d_lin_jobs_, 2 * sizeof(FrontJob));
+        if (hipHostMalloc((void**)&h_lin_jobs_, 2 * sizeof(FrontJob), hipHostMallocDefault) != hipSuccess)
+            VSHIP_THROW(Vship_OutOfRAM, "hipHostMalloc failed in SSIMU2 init");
+        std::memset(h_lin_jobs_, 0, 2 * sizeof(FrontJob));
+    }
+    for (int i = 0; i < 2; i++) {
+        const ImageSpec& sp = spec_[i];
+        if (!sp.resample) continue;
+        const int64_t W = sp.width, H = sp.height, tw = sp.target_w, th = sp.target_h;
+        dmalloc((void**)&lin_[i], (size_t)(3 * W * H) * sizeof(float));
+        const float* cur = lin_[i];
+        if (tw != W) {
+            dmalloc((void**)&rsz_h_[i], (size_t)(3 * tw * H) * sizeof(float));
+            cur = rsz_h_[i];
+        }
+        if (th != H) {
+            dmalloc((void**)&rsz_[i], (size_t)(3 * tw * th) * sizeof(float));
+            cur = rsz_[i];
+        }
+        FrontJob& lj = h_lin_jobs_[i];
+        lj.color[0] = i;
+        lj.width = (int32_t)W;
+        lj.height = (int32_t)H;
+        lj.tiles_x = (int32_t)((W + kFrontTileW - 1) / kFrontTileW);
+        for (int p = 0; p < 3; p++) {
+            j.plane[i][p] = reinterpret_cast<const uint8_t*>(cur + p * tw * th + sp.crop_top * tw + sp.crop_left);
+            j.stride[i][p] = tw * (int64_t)sizeof(float);
+        }
+        j.color[i] = 2;
+    }
 }
 
 Ssimu2Handler::~Ssimu2Handler()
@@ -384,8 +443,13 @@ Ssimu2Handler::~Ssimu2Handler()
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (int i = 0; i < 2; i++) {
         (void)hipFree(xyb_[i]);
+        (void)hipFree(lin_[i]);
+        (void)hipFree(rsz_h_[i]);
+        (void)hipFree(rsz_[i]);
         for (int p = 0; p < 3; p++) (void)hipFree(staging_[i][p]);
     }
+    (void)hipFree(d_lin_jobs_);
+    if (h_lin_jobs_) (void)hipHostFree(h_lin_jobs_);
     (void)hipFree(d_front_jobs_);
     (void)hipFree(d_colors_);
     (void)hipFree(d_geom_);
@@ -420,6 +484,23 @@ void Ssimu2Handler::ensure_staging(int image, const int64_t ls[3])
 
 void Ssimu2Handler::launch_all()
 {
+    if (d_lin_jobs_) {
+        VSHIP_HIP_CHECK(
+            hipMemcpyAsync(d_lin_jobs_, h_lin_jobs_, 2 * sizeof(FrontJob), hipMemcpyHostToDevice, stream_));
+        for (int i = 0; i < 2; i++) {
+            const ImageSpec& sp = spec_[i];
+            if (!sp.resample) continue;
+            const int W = (int)sp.width, H = (int)sp.height, tw = (int)sp.target_w, th = (int)sp.target_h;
+            const int tiles = h_lin_jobs_[i].tiles_x * ((H + kFrontTileH - 1) / kFrontTileH);
+            VSHIP_HIP_CHECK(launch_linear_rgb(d_lin_jobs_ + i, tiles, d_colors_, lin_[i], W, front_lds_, stream_));
+            const float* cur = lin_[i];
+            if (tw != W) {
+                VSHIP_HIP_CHECK(launch_resize_h(rsz_h_[i], cur, W, H, tw, stream_));
+                cur = rsz_h_[i];
+            }
+            if (th != H) VSHIP_HIP_CHECK(launch_resize_v(rsz_[i], cur, tw, H, th, stream_));
+        }
+    }
     VSHIP_HIP_CHECK(hipMemcpyAsync(d_front_jobs_, h_front_jobs_, sizeof(FrontJob), hipMemcpyHostToDevice, stream_));
     VSHIP_HIP_CHECK(launch_frontend<float>(d_front_jobs_, d_counts_, layout_.front_tiles_per_image, d_colors_, d_geom_,
                                            front_lds_, stream_));
@@ -431,11 +512,17 @@ void Ssimu2Handler::launch_all()
 void Ssimu2Handler::run(const uint8_t* const p1[3], const uint8_t* const p2[3], const int64_t ls1[3],
                         const int64_t ls2[3])
 {
-    for (int p = 0; p < 3; p++) {
-        h_front_jobs_->plane[0][p] = p1[p];
-        h_front_jobs_->stride[0][p] = ls1[p];
-        h_front_jobs_->plane[1][p] = p2[p];
-        h_front_jobs_->stride[1][p] = ls2[p];
+    const uint8_t* const* planes[2] = {p1, p2};
+    const int64_t* strides[2] = {ls1, ls2};
+    for (int i = 0; i < 2; i++) {
+        // resampled images enter through the linear-RGB job; the pyramid job
+        // keeps its fixed pointers to the resized planes
+        FrontJob& dst = spec_[i].resample ? h_lin_jobs_[i] : *h_front_jobs_;
+        const int k = spec_[i].resample ? 0 : i;
+        for (int p = 0; p < 3; p++) {
+            dst.plane[k][p] = planes[i][p];
+            dst.stride[k][p] = strides[i][p];
+        }
     }
     launch_all();
     VSHIP_HIP_CHECK(hipMemcpyAsync(h_score_, d_score_, sizeof(double), hipMemcpyDeviceToHost, stream_));

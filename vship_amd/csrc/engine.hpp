@@ -22,9 +22,17 @@ namespace vship {
 struct ImageSpec {
     FrontColor fc;
     int bytes;                    // bytes per sample
-    int64_t width, height;        // input size (level 0 when no crop/resize)
+    int64_t width, height;        // input size
     int64_t plane_w[3], plane_h[3];
+    bool resample;                // resize and/or crop before the pyramid
+    int64_t target_w, target_h;   // size after the resize
+    int64_t crop_left, crop_top;
+    int64_t final_w, final_h;     // size the metric sees (level 0)
 };
+
+// Colour description of linear BT.709 RGB float planes (the resampled path's
+// input to the pyramid kernel).
+FrontColor linear_rgb_color();
 
 ImageSpec make_image_spec(const Vship_Colorspace_t& cs);
 
@@ -96,6 +104,13 @@ private:
     double* d_score_ = nullptr;
     FrontJob* h_front_jobs_ = nullptr;         // pinned, one pair
     double* h_score_ = nullptr;                // pinned
+    // crop/resize path, per image: linear RGB at the input size, after the
+    // horizontal resize, after the vertical resize (3 planes each)
+    float* lin_[2] = {nullptr, nullptr};
+    float* rsz_h_[2] = {nullptr, nullptr};
+    float* rsz_[2] = {nullptr, nullptr};
+    FrontJob* d_lin_jobs_ = nullptr;           // one per image
+    FrontJob* h_lin_jobs_ = nullptr;           // pinned
     uint8_t* staging_[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
     size_t staging_cap_[2][3] = {{0, 0, 0}, {0, 0, 0}};
     bool have_run_ = false;

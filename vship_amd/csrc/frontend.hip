@@ -217,8 +217,11 @@ VSHIP_INLINE void down_lds(const float* src, int cols, int rows, int lx, int ly,
     }
 }
 
-VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc, const PairGeom& geo, int tile_x,
-                               int tile_y, float* lds, TileXyb& res)
+// Linear BT.709 RGB of the thread's 2x2 block of image k: sample decode,
+// range, chroma upsampling (LDS), matrix, EOTF, linear-light matrix and
+// primaries. Ends with the LDS free for reuse only after the caller syncs.
+VSHIP_INLINE void to_linear(const FrontJob& job, int k, const FrontColor& fc, int tile_x, int tile_y, float* lds,
+                            float P[3][kBH][kBW])
 {
     const uint8_t* const* plane = job.plane[k];
     const int64_t* pstride = job.stride[k];
@@ -230,7 +233,7 @@ VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc,
     const RangeXform r0 = {fc.range0[0], fc.range0[1]};
     const RangeXform r12 = {fc.range12[0], fc.range12[1]};
 
-    float P[3][kBH][kBW];  // plane values, then linear R, G, B of the block
+    // P: plane values, then linear R, G, B of the block
     const bool sub = fc.subw || fc.subh;
     switch (fc.sample) {
     case kSampleU8:
@@ -366,6 +369,18 @@ VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc,
 #pragma unroll
             for (int i = 0; i < kBW; i++) mat3_apply(fc.post, P[0][j][i], P[1][j][i], P[2][j][i]);
     }
+}
+
+VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc, const PairGeom& geo, int tile_x,
+                               int tile_y, float* lds, TileXyb& res)
+{
+    const int t = threadIdx.x;
+    const int tx = t % kTX, ty = t / kTX;
+    const int W = job.width, H = job.height;
+    const int x0 = tile_x * kFrontTileW + kBW * tx;
+    const int y0 = tile_y * kFrontTileH + kBH * ty;
+    float P[3][kBH][kBW];
+    to_linear(job, k, fc, tile_x, tile_y, lds, P);
 
     // ---- level 0: Y only
 #pragma unroll
@@ -549,7 +564,42 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
     }
 }
 
+// Linear RGB of image 0 of one job, stored as three W x H planes with row
+// pitch `pitch` elements (the input of the crop/resize path; reference
+// vshipColor.hpp:62-86 before its resize step).
+__global__ __launch_bounds__(kFrontThreads)
+void linear_rgb_kernel(const FrontJob* __restrict__ job_ptr, const FrontColor* __restrict__ colors,
+                       float* __restrict__ out, int64_t pitch)
+{
+    extern __shared__ float lds[];
+    const FrontJob& job = *job_ptr;
+    const int tile_x = blockIdx.x % job.tiles_x, tile_y = blockIdx.x / job.tiles_x;
+    float P[3][kBH][kBW];
+    to_linear(job, 0, colors[job.color[0]], tile_x, tile_y, lds, P);
+    const int t = threadIdx.x;
+    const int x0 = tile_x * kFrontTileW + kBW * (t % kTX);
+    const int y0 = tile_y * kFrontTileH + kBH * (t / kTX);
+    const int64_t plane_elems = pitch * job.height;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int j = 0; j < kBH; j++)
+#pragma unroll
+            for (int i = 0; i < kBW; i++)
+                if (x0 + i < job.width && y0 + j < job.height)
+                    out[c * plane_elems + (int64_t)(y0 + j) * pitch + x0 + i] = P[c][j][i];
+}
+
 }  // namespace
+
+hipError_t launch_linear_rgb(const FrontJob* job, int tiles, const FrontColor* colors, float* out, int64_t pitch,
+                             size_t lds_bytes, hipStream_t stream)
+{
+    if (tiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(linear_rgb_kernel, dim3(tiles), dim3(kFrontThreads), lds_bytes, stream, job, colors, out,
+                       pitch);
+    return hipGetLastError();
+}
 
 size_t frontend_lds_bytes(const FrontColor& fc)
 {

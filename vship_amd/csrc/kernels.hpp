@@ -16,6 +16,17 @@ template <typename StoreT>
 hipError_t launch_frontend(const FrontJob* jobs, const int* counts, int grid, const FrontColor* colors,
                            const PairGeom* geoms, size_t lds_bytes, hipStream_t stream);
 
+// Linear RGB of image 0 of `job` (tiles = job's 64x32 tile count) into three
+// planes of job.width x job.height floats, row pitch `pitch` elements.
+hipError_t launch_linear_rgb(const FrontJob* job, int tiles, const FrontColor* colors, float* out, int64_t pitch,
+                             size_t lds_bytes, hipStream_t stream);
+
+// Separable cubic-Hermite (Catmull-Rom) resize of three float planes,
+// reference src/gpuColorToLinear/resize.hpp:26-65: horizontal sw -> rw on h
+// rows, then vertical sh -> rh on w columns (dense planes).
+hipError_t launch_resize_h(float* dst, const float* src, int sw, int h, int rw, hipStream_t stream);
+hipError_t launch_resize_v(float* dst, const float* src, int w, int sh, int rh, hipStream_t stream);
+
 // Blur/map tile shapes: config 0 = 64x32 (default), 1 = 64x16, 2 = 32x64,
 // 3 = 32x32, 4 = 64x64 (FP32 VALU kernel); 5 = 64x48 (matrix-core kernel,
 // blur_mfma.hip).
