@@ -13,7 +13,8 @@ def pytest_configure(config):
     lib = os.path.join(ROOT, "vship_amd", "libvship_amd.so")
     olib = os.path.join(ROOT, "oracle", "liboracle.so")
     mlib = os.path.join(ROOT, "tests", "vs_host", "libmock_vs.so")
-    if not (os.path.exists(lib) and os.path.exists(olib) and os.path.exists(mlib)):
+    drv = os.path.join(ROOT, "vship_amd", "ffvship_amd")
+    if not all(os.path.exists(p) for p in (lib, olib, mlib, drv)):
         import __graft_entry__
         __graft_entry__.build()
 
