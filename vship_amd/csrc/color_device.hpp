@@ -64,24 +64,27 @@ __device__ inline float catmull_rom(float pm1, float p0, float p1, float p2, flo
     return r;
 }
 
-// x^g for x > 0 from the hardware log2/exp2 (v_log_f32, v_exp_f32).
+// x^g (g > 0) for x >= 0 from the hardware log2/exp2 (v_log_f32, v_exp_f32),
+// without a branch: log2(+0) = -inf and exp2(-inf) = +0, so x = 0 gives 0;
+// negative and NaN inputs are taken as 0 by the max.
 __device__ inline float pow_pos(float x, float g)
 {
-    return x > 0.0f ? __builtin_amdgcn_exp2f(g * __builtin_amdgcn_logf(x)) : 0.0f;
+    return __builtin_amdgcn_exp2f(g * __builtin_amdgcn_logf(fmaxf(x, 0.0f)));
 }
 
+// sign(x) * |x|^g, branch-free (the magnitude takes x's sign bit).
 __device__ inline float pow_signed(float x, float g)
 {
-    return x < 0.0f ? -pow_pos(-x, g) : pow_pos(x, g);
+    return copysignf(__builtin_amdgcn_exp2f(g * __builtin_amdgcn_logf(fabsf(x))), x);
 }
 
 // Cube root of x >= 0 as exp2(log2(x) / 3) from the hardware log2/exp2
 // (each about 1 ulp): relative error of a few 1e-7 over the range the opsin
-// mix produces, far below what the score can resolve.
+// mix produces, far below what the score can resolve. Non-positive and NaN
+// inputs give 0 (the clamp of makeXYB.hpp:36-38), branch-free as in pow_pos.
 __device__ inline float cbrt_pos(float x)
 {
-    if (!(x > 0.0f)) return 0.0f;
-    return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.0f / 3.0f));
+    return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(fmaxf(x, 0.0f)) * (1.0f / 3.0f));
 }
 
 // Per-component EOTFs, reference src/gpuColorToLinear/transferToLinear.hpp:
@@ -90,16 +93,21 @@ __device__ inline float cbrt_pos(float x)
 __device__ inline float eotf(float a, int trc)
 {
     switch (trc) {
-    case 13: /* sRGB */
-        if (a < 0.0f) return (a < -0.04045f) ? -pow_pos((-a + 0.055f) * (1.0f / 1.055f), 2.4f) : a * (1.0f / 12.92f);
-        return (a > 0.04045f) ? pow_pos((a + 0.055f) * (1.0f / 1.055f), 2.4f) : a * (1.0f / 12.92f);
+    case 13: { /* sRGB, sign-symmetric; both branches evaluated, one selected */
+        const float ax = fabsf(a);
+        const float pw = pow_pos((ax + 0.055f) * (1.0f / 1.055f), 2.4f);
+        return copysignf(ax > 0.04045f ? pw : ax * (1.0f / 12.92f), a);
+    }
     case 1: /* BT709 */ return pow_signed(a, 2.4f);
     case 4: /* BT470_M */ return pow_signed(a, 2.2f);
     case 5: /* BT470_BG */
     case 6: /* BT601 */ return pow_signed(a, 2.8f);
     case 7: /* ST240_M */
-        if (a >= 0.0f) return (a < 0.0228f) ? 4.0f * a : 1.1115f * pow_pos(a, 0.45f) - 0.1115f;
-        return (a > -0.0228f) ? 4.0f * a : -1.1115f * pow_pos(-a, 0.45f) + 0.1115f;
+    {
+        const float ax = fabsf(a);
+        const float pw = fmaf(1.1115f, pow_pos(ax, 0.45f), -0.1115f);
+        return copysignf(ax < 0.0228f ? 4.0f * ax : pw, a);
+    }
     case 17: /* ST428 */ return pow_signed(a, 2.6f) * (float)(52.37 / 48.0);
     case 16: { /* PQ */
         const float m = 78.84375f, n = 0.1593017578125f;

@@ -56,6 +56,16 @@ VSHIP_INLINE float cr_apply(const CrWeights& w, float pm1, float p0, float p1, f
     return fmaf(w.w0, pm1, fmaf(w.w1, p0, fmaf(w.w2, p1, w.w3 * p2)));
 }
 
+// Sample p0 itself on a cosited position (copy), else the interpolation.
+// The four taps are read unconditionally (they lie inside the staged halo
+// either way) and the result selected, so no lane diverges.
+VSHIP_INLINE float cr_pick(bool copy, const CrWeights& w, const float* p, int pitch)
+{
+    const float pm1 = p[-pitch], p0 = p[0], p1 = p[pitch], p2 = p[2 * pitch];
+    const float v = cr_apply(w, pm1, p0, p1, p2);
+    return copy ? p0 : v;
+}
+
 // Segment of full-resolution coordinate X for decimation 2^sub (sub >= 1):
 // X maps to segment [k, k+1] at parameter t. Cosited: sample k sits on
 // X = k * 2^sub (kernels *LeftUpscale*, *TopUpscale*); centred: on
@@ -299,8 +309,7 @@ VSHIP_INLINE void to_linear(const FrontJob& job, int k, const FrontColor& fc, in
             for (int p = 0; p < 2; p++) {
                 for (int r = t / kFrontTileW; r < crows; r += kFrontThreads / kFrontTileW) {
                     const float* row = s_chroma + p * plane_elems + r * ccols;
-                    s_mid[(p * crows + r) * kFrontTileW + X] =
-                        copy ? row[c] : cr_apply(cw4, row[c - 1], row[c], row[c + 1], row[c + 2]);
+                    s_mid[(p * crows + r) * kFrontTileW + X] = cr_pick(copy, cw4, row + c, 1);
                 }
             }
             __syncthreads();
@@ -333,7 +342,7 @@ VSHIP_INLINE void to_linear(const FrontJob& job, int k, const FrontColor& fc, in
                     if (fc.subh == 0) {
                         const float* row = s_chroma + p * plane_elems + (y0 + j - cy0) * ccols;
                         const int c = kx[i] - cx0;
-                        val = cpx[i] ? row[c] : cr_apply(wx[i], row[c - 1], row[c], row[c + 1], row[c + 2]);
+                        val = cr_pick(cpx[i], wx[i], row + c, 1);
                     } else {
                         const int r = ky[j] - cy0;
                         const float* colbase;
@@ -345,9 +354,7 @@ VSHIP_INLINE void to_linear(const FrontJob& job, int k, const FrontColor& fc, in
                             colbase = s_chroma + p * plane_elems + (x0 + i - cx0);
                             pitch = ccols;
                         }
-                        val = cpy[j] ? colbase[r * pitch]
-                                     : cr_apply(wy[j], colbase[(r - 1) * pitch], colbase[r * pitch],
-                                                colbase[(r + 1) * pitch], colbase[(r + 2) * pitch]);
+                        val = cr_pick(cpy[j], wy[j], colbase + r * pitch, pitch);
                     }
                     P[1 + p][j][i] = val;
                 }
