@@ -56,9 +56,10 @@ typedef enum {
     VshipAMD_StorageFP16 = 1,
 } VshipAMD_Storage;
 
-/* Tuning of the batched kernels: blur tile shape (0: 64x32, 1: 64x16,
- * 2: 32x64, 3: 32x32, 4: 64x64) and persistent workgroups per CU of the
- * blur and front-end kernels. */
+/* Tuning of the batched kernels: blur tile shape (FP32 VALU kernel 0: 64x32,
+ * 1: 64x16, 2: 32x64, 3: 32x32, 4: 64x64; matrix-core kernels 5: 64x48
+ * tiles, 6: 64x64, 7: 64x128, 8: 64x256 streamed strips) and persistent
+ * workgroups per CU of the blur and front-end kernels. */
 typedef struct VshipAMD_BatchTuning {
     int blur_config;
     int blur_waves;

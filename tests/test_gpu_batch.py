@@ -11,6 +11,7 @@ from tests import cases
 pytestmark = pytest.mark.gpu
 
 SIZES = [(131, 75), (64, 64), (200, 117), (17, 9), (1, 1), (320, 180), (97, 61), (640, 360)]
+BLUR_CONFIGS = 9  # 0-4 FP32 VALU tiles, 5 matrix-core 64x48 tiles, 6-8 streamed matrix-core strips
 
 
 def _say(*a):
@@ -106,26 +107,29 @@ def test_autotune_keeps_results(pairs):
         base = b.compute(*pairs.args())
         tuning, ms = b.autotune(*pairs.args(), reps=1)
         _say("autotune", tuning, ms)
-        assert 0 <= tuning["blur_config"] < 6 and ms > 0
+        assert 0 <= tuning["blur_config"] < BLUR_CONFIGS and ms > 0
         assert b.tuning() == tuning
         tuned = b.compute(*pairs.args())
         # tile shapes change the per-tile centring constant and the order of
         # the float tile sums, so scores move at float-rounding level only
-        for cfg in range(6):
+        for cfg in range(BLUR_CONFIGS):
             b.set_tuning(cfg, 2, 2)
             np.testing.assert_allclose(b.compute(*pairs.args()), base, rtol=0, atol=1e-4)
     np.testing.assert_allclose(tuned, base, rtol=0, atol=1e-4)
 
 
+@pytest.mark.parametrize("config", [5, 6, 7, 8])
 @pytest.mark.parametrize("storage,bar", [(S.STORAGE_FP32, 0.005), (S.STORAGE_FP16, 0.01)])
-def test_matrix_core_blur_vs_oracle(pairs, storage, bar):
-    """Blur config 5 (banded Toeplitz products on the matrix cores, FP16
-    hi/lo operands) against the FP64 oracle and the FP32 VALU kernel, on
-    every size of the fixture (1x1 and 17x9 included); identity stays 100."""
+def test_matrix_core_blur_vs_oracle(pairs, storage, bar, config):
+    """Matrix-core blur kernels (config 5: 64x48 tiles with banded Toeplitz
+    products; 6-8: 64-column strips of 64/128/256 rows with the horizontal
+    results fed to the vertical products from registers; FP16 hi/lo operands)
+    against the FP64 oracle and the FP32 VALU kernel, on every size of the
+    fixture (1x1 and 17x9 included); identity stays 100."""
     with S.SSIMU2Batch(16, storage) as b:
         b.set_tuning(0, 2, 2)
         valu = b.compute(*pairs.args())
-        b.set_tuning(5, 2, 2)
+        b.set_tuning(config, 2, 2)
         got = b.compute(*pairs.args())
         cs, _, p1, _, st, _ = pairs.args()
         ident = b.compute(cs, cs, p1, p1, st, st)
@@ -133,7 +137,7 @@ def test_matrix_core_blur_vs_oracle(pairs, storage, bar):
     for i, cs_i in enumerate(pairs.cs):
         ref, dist = pairs.host[i]
         want, _ = oracle.ssimu2(cs_i, ref, pairs.strides[i], cs_i, dist, pairs.strides[i])
-        _say(f"mfma blur {cs_i.width}x{cs_i.height}: {got[i]:.6f} valu {valu[i]:.6f} oracle {want:.6f}")
+        _say(f"mfma blur config {config} {cs_i.width}x{cs_i.height}: {got[i]:.6f} valu {valu[i]:.6f} oracle {want:.6f}")
         worst = max(worst, abs(got[i] - want))
         assert abs(got[i] - valu[i]) <= bar
     assert worst <= bar, worst

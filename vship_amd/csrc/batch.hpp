@@ -25,8 +25,8 @@
 namespace vship {
 
 struct BatchTuning {
-    int blur_config = kDefaultBlurConfig;  // blur tile shape (blur_tile_shape)
-    int blur_waves = 2;      // persistent blur workgroups per CU
+    int blur_config = kDefaultBatchBlurConfig;  // blur tile shape (blur_tile_shape)
+    int blur_waves = kDefaultBatchBlurWaves;    // persistent blur workgroups per CU
     int front_waves = 4;     // persistent front-end workgroups per CU
 };
 

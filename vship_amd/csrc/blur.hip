@@ -295,11 +295,14 @@ BlurTile blur_tile_shape(int config)
     case 3: return {32, 32};
     case 4: return {64, 64};
     case 5: return {64, 48};
+    case 6: return {64, 64};
+    case 7: return {64, 128};
+    case 8: return {64, 256};
     default: return {64, 32};
     }
 }
 
-int blur_tile_configs() { return 6; }
+int blur_tile_configs() { return 9; }
 
 int blur_grid_for(int total_tiles)
 {
@@ -318,6 +321,9 @@ hipError_t launch_blur(int config, const MapJob* jobs, const int* counts, int gr
     case 3: return launch_one<StoreT, 32, 32>(jobs, counts, grid, taps, partials, stream);
     case 4: return launch_one<StoreT, 64, 64>(jobs, counts, grid, taps, partials, stream);
     case 5: return launch_blur_mfma<StoreT>(jobs, counts, grid, taps, partials, stream);
+    case 6: return launch_blur_strip<StoreT>(4, jobs, counts, grid, taps, partials, stream);
+    case 7: return launch_blur_strip<StoreT>(8, jobs, counts, grid, taps, partials, stream);
+    case 8: return launch_blur_strip<StoreT>(16, jobs, counts, grid, taps, partials, stream);
     default: return launch_one<StoreT, 64, 32>(jobs, counts, grid, taps, partials, stream);
     }
 }

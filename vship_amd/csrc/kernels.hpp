@@ -27,9 +27,10 @@ hipError_t launch_linear_rgb(const FrontJob* job, int tiles, const FrontColor* c
 hipError_t launch_resize_h(float* dst, const float* src, int sw, int h, int rw, hipStream_t stream);
 hipError_t launch_resize_v(float* dst, const float* src, int w, int sh, int rh, hipStream_t stream);
 
-// Blur/map tile shapes: config 0 = 64x32 (default), 1 = 64x16, 2 = 32x64,
+// Blur/map tile shapes: config 0 = 64x32, 1 = 64x16, 2 = 32x64,
 // 3 = 32x32, 4 = 64x64 (FP32 VALU kernel); 5 = 64x48 (matrix-core kernel,
-// blur_mfma.hip).
+// blur_mfma.hip); 6 = 64x64, 7 = 64x128, 8 = 64x256 (streamed matrix-core
+// kernel, blur_strip.hip).
 struct BlurTile {
     int tw, th;
 };
@@ -48,6 +49,11 @@ hipError_t launch_blur(int config, const MapJob* jobs, const int* counts, int gr
 template <typename StoreT>
 hipError_t launch_blur_mfma(const MapJob* jobs, const int* counts, int grid, const BlurTaps& taps, float* partials,
                             hipStream_t stream);
+
+// Streamed matrix-core variant (items of 64 x 16*segb outputs; configs 6-8).
+template <typename StoreT>
+hipError_t launch_blur_strip(int segb, const MapJob* jobs, const int* counts, int grid, const BlurTaps& taps,
+                             float* partials, hipStream_t stream);
 
 hipError_t launch_score(const ScoreJob* pairs, const int* counts, int grid, const MapJob* jobs,
                         const float* partials, double* features, double* scores, hipStream_t stream);

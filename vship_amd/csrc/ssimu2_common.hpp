@@ -23,7 +23,12 @@ constexpr float kC2 = 0.0009f;    // SSIM constant (score.hpp:272-273)
 // Front-end tile: 64x32 level-0 pixels (whole 4x2 blocks at level 4), 512
 // threads, each owning a 2x2 block.
 constexpr int kXcds = 8;  // MI355X: workgroups are dispatched round-robin over 8 XCDs, each with its own L2
-constexpr int kDefaultBlurConfig = 5;  // blur tile config: 5 = the matrix-core kernel, fastest measured on MI355X
+// Blur tile configs (blur_tile_shape): the single-pair handler uses 64x128
+// strips (7) and the batch 64x256 strips (8) with 3 workgroups per CU, the
+// fastest measured on MI355X for one 1080p pair and for 240 of them.
+constexpr int kDefaultBlurConfig = 7;
+constexpr int kDefaultBatchBlurConfig = 8;
+constexpr int kDefaultBatchBlurWaves = 3;
 constexpr int kFrontTileW = 64;
 constexpr int kFrontTileH = 32;
 constexpr int kFrontThreads = 512;

@@ -46,7 +46,9 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--mib", type=int, default=256)
     ap.add_argument("--storage", choices=["fp16", "fp32"], default="fp16")
-    ap.add_argument("--blur-config", type=int, default=None, help="batch: blur tile config (5 = matrix cores)")
+    ap.add_argument("--blur-config", type=int, default=None, help="batch: blur tile config (5-8 = matrix cores)")
+    ap.add_argument("--blur-waves", type=int, default=None, help="batch: blur workgroups per CU")
+    ap.add_argument("--front-waves", type=int, default=None, help="batch: front-end workgroups per CU")
     a = ap.parse_args(argv)
     if a.mode == "copy":
         nbytes = a.mib << 20
@@ -69,9 +71,11 @@ def main(argv=None):
     else:
         storage = S.STORAGE_FP16 if a.storage == "fp16" else S.STORAGE_FP32
         with S.SSIMU2Batch(a.pairs, storage) as b:
-            if a.blur_config is not None:
+            if a.blur_config is not None or a.blur_waves is not None or a.front_waves is not None:
                 t = b.tuning()
-                b.set_tuning(a.blur_config, t["blur_waves"], t["front_waves"])
+                b.set_tuning(t["blur_config"] if a.blur_config is None else a.blur_config,
+                             t["blur_waves"] if a.blur_waves is None else a.blur_waves,
+                             t["front_waves"] if a.front_waves is None else a.front_waves)
             args = ([cs] * a.pairs, [cs] * a.pairs, [p[0] for p in ptrs], [p[1] for p in ptrs], [st] * a.pairs,
                     [st] * a.pairs)
             for _ in range(a.reps):
