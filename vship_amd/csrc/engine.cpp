@@ -158,6 +158,32 @@ int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
 }  // namespace
 
+// Catmull-Rom weights (the cubic Hermite of resize.hpp:5-24 with tangents
+// (p[i+1] - p[i-1]) / 2) of the two output phases of a 2x-upsampled chroma
+// axis, over the window of chroma samples m-2..m+2 around m = X >> 1 for
+// output X = 2m + phase. Cosited samples sit on even X (phase 0 copies,
+// phase 1 is t = 0.5 between m and m+1, chromaUpsample.hpp:28-52); centred
+// samples sit on X = 2k + 0.5 (phase 0 is t = 0.75 between m-1 and m, phase
+// 1 is t = 0.25 between m and m+1, chromaUpsample.hpp:54-149).
+static void phase_weights(bool cosited, float w[2][5])
+{
+    for (int ph = 0; ph < 2; ph++) {
+        for (int q = 0; q < 5; q++) w[ph][q] = 0.0f;
+        const int k = cosited ? 0 : (ph == 0 ? -1 : 0);  // segment start, relative to m
+        if (cosited && ph == 0) {
+            w[ph][k + 2] = 1.0f;
+            continue;
+        }
+        const float t = cosited ? 0.5f : (ph == 0 ? 0.75f : 0.25f);
+        const float t2 = t * t, t3 = t2 * t;
+        // taps at chroma m+k-1 .. m+k+2 = window entries k+1 .. k+4
+        w[ph][k + 1] = 0.5f * (-t3 + 2.0f * t2 - t);
+        w[ph][k + 2] = 0.5f * (3.0f * t3 - 5.0f * t2 + 2.0f);
+        w[ph][k + 3] = 0.5f * (-3.0f * t3 + 4.0f * t2 + t);
+        w[ph][k + 4] = 0.5f * (t3 - t2);
+    }
+}
+
 ImageSpec make_image_spec(const Vship_Colorspace_t& cs)
 {
     ImageSpec s;
@@ -222,6 +248,8 @@ ImageSpec make_image_spec(const Vship_Colorspace_t& cs)
     fc.h_cosited = (cs.chromaLocation == Vship_ChromaLoc_Left || cs.chromaLocation == Vship_ChromaLoc_TopLeft);
     fc.v_cosited = (cs.chromaLocation == Vship_ChromaLoc_Top || cs.chromaLocation == Vship_ChromaLoc_TopLeft);
     fc.transfer = cs.transferFunction;
+    phase_weights(fc.h_cosited != 0, fc.hw);
+    phase_weights(fc.v_cosited != 0, fc.vw);
     const RangeXform r0 = make_range(cs.range, bits, false);
     const RangeXform r12 = make_range(cs.range, bits, cs.colorFamily == Vship_ColorYUV);
     fc.range0[0] = r0.mul; fc.range0[1] = r0.add;

@@ -89,19 +89,42 @@ VSHIP_INLINE void seg_of(int X, int sub, bool cosited, int& k, float& t, bool& c
     }
 }
 
+typedef const __attribute__((address_space(1))) uint8_t GByte;
+typedef const __attribute__((address_space(1))) uint16_t GU16;
+typedef const __attribute__((address_space(1))) uint32_t GU32;
+
+// One raw sample as float. Plane pointers come from job tables, so they are
+// read through the global address space (flat loads would also count against
+// the LDS wait counter). Samples of planes whose base and row stride are
+// multiples of the sample size are read with one load, others byte by byte.
 template <int KIND>
-VSHIP_INLINE float load_kind(const uint8_t* p, uint32_t mask)
+VSHIP_INLINE float load_kind(const uint8_t* p8, uint32_t mask, bool aligned)
 {
+    GByte* p = (GByte*)p8;
     if constexpr (KIND == kSampleU8) {
         return (float)p[0];
     } else if constexpr (KIND == kSampleU16) {
-        return (float)(((uint32_t)p[0] | ((uint32_t)p[1] << 8)) & mask);
+        const uint32_t v = aligned ? (uint32_t)*(GU16*)p : ((uint32_t)p[0] | ((uint32_t)p[1] << 8));
+        return (float)(v & mask);
     } else if constexpr (KIND == kSampleHalf) {
-        return __half2float(__ushort_as_half((uint16_t)((uint32_t)p[0] | ((uint32_t)p[1] << 8))));
+        const uint32_t v = aligned ? (uint32_t)*(GU16*)p : ((uint32_t)p[0] | ((uint32_t)p[1] << 8));
+        return __half2float(__ushort_as_half((uint16_t)v));
     } else {
-        return __uint_as_float((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) |
-                               ((uint32_t)p[3] << 24));
+        return __uint_as_float(aligned ? *(GU32*)p
+                                       : ((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) |
+                                          ((uint32_t)p[3] << 24)));
     }
+}
+
+template <int KIND>
+constexpr int kind_bytes();
+
+// Whether every sample of a plane (base + y * stride + x * bytes) is aligned
+// to its size.
+template <int KIND>
+VSHIP_INLINE bool plane_aligned(const uint8_t* base, int64_t stride)
+{
+    return (((uintptr_t)base | (uintptr_t)stride) & (uintptr_t)(kind_bytes<KIND>() - 1)) == 0;
 }
 
 template <int KIND>
@@ -112,35 +135,60 @@ template <int KIND>
 VSHIP_INLINE void load_block(const uint8_t* base, int64_t stride, int x0, int y0, int W, int H, uint32_t mask,
                              const RangeXform& rx, float out[kBH][kBW])
 {
+    const bool aligned = plane_aligned<KIND>(base, stride);
 #pragma unroll
     for (int j = 0; j < kBH; j++) {
         const uint8_t* row = base + (int64_t)min(y0 + j, H - 1) * stride;
 #pragma unroll
         for (int i = 0; i < kBW; i++)
-            out[j][i] = apply_range(load_kind<KIND>(row + min(x0 + i, W - 1) * kind_bytes<KIND>(), mask), rx);
+            out[j][i] =
+                apply_range(load_kind<KIND>(row + min(x0 + i, W - 1) * kind_bytes<KIND>(), mask, aligned), rx);
     }
 }
 
 // Both chroma planes of the tile (with halo) into LDS, clamped coordinates.
+// Every thread issues all of its loads (at most kChromaIters per plane: the
+// largest staged plane is 68 x 20 or 36 x 36 samples) before the first one is
+// used, so the tile waits for one memory round trip instead of one per row.
+constexpr int kChromaIters = 3;
+static_assert(kChromaIters * kFrontThreads >= 68 * 20 && kChromaIters * kFrontThreads >= 36 * 36,
+              "chroma staging items per thread");
+
 template <int KIND>
 VSHIP_INLINE void stage_chroma(const uint8_t* const* plane, const int64_t* pstride, uint32_t mask,
                                const RangeXform& rx, float* s_chroma, int ccols, int crows, int cx0, int cy0, int cw,
                                int ch)
 {
     const int t = threadIdx.x;
-    const int tx = t % kTX, ty = t / kTX;
+    const int plane_elems = ccols * crows;
+    // row = e / ccols from a float reciprocal: (e + 0.5) / ccols lies at least
+    // 0.5 / ccols (>= 0.007) from an integer, far beyond the float error
+    const float inv_cols = 1.0f / (float)ccols;
+    float v[2][kChromaIters];
 #pragma unroll
     for (int p = 0; p < 2; p++) {
         const uint8_t* base = plane[1 + p];
         const int64_t stride = pstride[1 + p];
-        float* dst = s_chroma + p * ccols * crows;
-        for (int r = ty; r < crows; r += kFrontThreads / kTX) {
-            const uint8_t* row = base + (int64_t)clampi(cy0 + r, 0, ch - 1) * stride;
-            for (int c = tx; c < ccols; c += kTX)
-                dst[r * ccols + c] =
-                    apply_range(load_kind<KIND>(row + clampi(cx0 + c, 0, cw - 1) * kind_bytes<KIND>(), mask), rx);
+        const bool aligned = plane_aligned<KIND>(base, stride);
+#pragma unroll
+        for (int it = 0; it < kChromaIters; it++) {
+            const int e = t + it * kFrontThreads;
+            v[p][it] = 0.0f;
+            if (e < plane_elems) {
+                const int r = (int)(((float)e + 0.5f) * inv_cols);
+                const int c = e - r * ccols;
+                const uint8_t* row = base + (int64_t)clampi(cy0 + r, 0, ch - 1) * stride;
+                v[p][it] = load_kind<KIND>(row + clampi(cx0 + c, 0, cw - 1) * kind_bytes<KIND>(), mask, aligned);
+            }
         }
     }
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+        for (int it = 0; it < kChromaIters; it++) {
+            const int e = t + it * kFrontThreads;
+            if (e < plane_elems) s_chroma[p * plane_elems + e] = apply_range(v[p][it], rx);
+        }
 }
 
 // EOTF of the thread's block, one code path per transfer.
@@ -227,6 +275,82 @@ VSHIP_INLINE void down_lds(const float* src, int cols, int rows, int lx, int ly,
     }
 }
 
+// Chroma of the thread's 2x2 block when no axis is subsampled more than 2x:
+// both output phases of a 2x axis have tile-independent Catmull-Rom weights
+// (FrontColor::hw, vw) over the 5 chroma samples around the block's own
+// chroma sample, so the upsampling is a 5-tap dot product per output and no
+// lane computes segment positions. With both axes subsampled the horizontal
+// pass runs over every staged row first (like the reference,
+// chromaUpsample.hpp:165-202), into s_mid with row pitch kMidPitch.
+constexpr int kMidPitch = kFrontTileW + 2;
+
+VSHIP_INLINE float dot5(const float w[5], const float v[5])
+{
+    return fmaf(w[0], v[0], fmaf(w[1], v[1], fmaf(w[2], v[2], fmaf(w[3], v[3], w[4] * v[4]))));
+}
+
+VSHIP_INLINE void chroma_2x(const FrontColor& fc, const float* s_chroma, float* s_mid, int ccols, int crows,
+                            bool both, float P[3][kBH][kBW])
+{
+    const int t = threadIdx.x;
+    const int tx = t % kTX, ty = t / kTX;
+    const int plane_elems = ccols * crows;
+    if (both) {
+        // lane pair (2m, 2m + 1) of tile columns from staged columns m .. m + 4
+        const int m = t & 31;
+        for (int e = t >> 5; e < 2 * crows; e += kFrontThreads / 32) {
+            const int p = e >= crows ? 1 : 0, r = e - p * crows;
+            const float* src = s_chroma + p * plane_elems + r * ccols + m;
+            float v[5];
+#pragma unroll
+            for (int q = 0; q < 5; q++) v[q] = src[q];
+            *reinterpret_cast<float2*>(s_mid + (p * crows + r) * kMidPitch + 2 * m) =
+                make_float2(dot5(fc.hw[0], v), dot5(fc.hw[1], v));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            float2 v2[5];
+#pragma unroll
+            for (int q = 0; q < 5; q++)
+                v2[q] = *reinterpret_cast<const float2*>(s_mid + (p * crows + ty + q) * kMidPitch + 2 * tx);
+            const float v0[5] = {v2[0].x, v2[1].x, v2[2].x, v2[3].x, v2[4].x};
+            const float v1[5] = {v2[0].y, v2[1].y, v2[2].y, v2[3].y, v2[4].y};
+#pragma unroll
+            for (int j = 0; j < kBH; j++) {
+                P[1 + p][j][0] = dot5(fc.vw[j], v0);
+                P[1 + p][j][1] = dot5(fc.vw[j], v1);
+            }
+        }
+    } else if (fc.subw) {
+        // 2x horizontal only: rows y0 + j are staged rows 2ty + j + 2
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int j = 0; j < kBH; j++) {
+                const float* src = s_chroma + p * plane_elems + (2 * ty + j + 2) * ccols + tx;
+                float v[5];
+#pragma unroll
+                for (int q = 0; q < 5; q++) v[q] = src[q];
+                P[1 + p][j][0] = dot5(fc.hw[0], v);
+                P[1 + p][j][1] = dot5(fc.hw[1], v);
+            }
+    } else {
+        // 2x vertical only: columns x0 + i are staged columns 2tx + i + 2
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int i = 0; i < kBW; i++) {
+                const float* src = s_chroma + p * plane_elems + ty * ccols + 2 * tx + i + 2;
+                float v[5];
+#pragma unroll
+                for (int q = 0; q < 5; q++) v[q] = src[q * ccols];
+                P[1 + p][0][i] = dot5(fc.vw[0], v);
+                P[1 + p][1][i] = dot5(fc.vw[1], v);
+            }
+    }
+}
+
 // Linear BT.709 RGB of the thread's 2x2 block of image k: sample decode,
 // range, chroma upsampling (LDS), matrix, EOTF, linear-light matrix and
 // primaries. Ends with the LDS free for reuse only after the caller syncs.
@@ -297,7 +421,9 @@ VSHIP_INLINE void to_linear(const FrontJob& job, int k, const FrontColor& fc, in
         }
         __syncthreads();
         const bool both = fc.subw && fc.subh;
-        if (both) {
+        if (fc.subw <= 1 && fc.subh <= 1) {
+            chroma_2x(fc, s_chroma, s_mid, ccols, crows, both, P);
+        } else if (both) {
             // horizontal pass over every staged chroma row first, like the
             // reference (chromaUpsample.hpp:165-202)
             const int X = t & (kFrontTileW - 1);
@@ -314,51 +440,53 @@ VSHIP_INLINE void to_linear(const FrontJob& job, int k, const FrontColor& fc, in
             }
             __syncthreads();
         }
-        // segment parameters of the thread's columns and rows
-        int kx[kBW], ky[kBH];
-        CrWeights wx[kBW], wy[kBH];
-        bool cpx[kBW], cpy[kBH];
-#pragma unroll
-        for (int i = 0; i < kBW; i++) {
-            float tt = 0.0f;
-            if (fc.subw) seg_of(x0 + i, fc.subw, fc.h_cosited != 0, kx[i], tt, cpx[i]);
-            else { kx[i] = x0 + i; cpx[i] = true; }
-            wx[i] = cr_weights(tt);
-        }
-#pragma unroll
-        for (int j = 0; j < kBH; j++) {
-            float tt = 0.0f;
-            if (fc.subh) seg_of(y0 + j, fc.subh, fc.v_cosited != 0, ky[j], tt, cpy[j]);
-            else { ky[j] = y0 + j; cpy[j] = true; }
-            wy[j] = cr_weights(tt);
-        }
-#pragma unroll
-        for (int p = 0; p < 2; p++) {
-#pragma unroll
-            for (int j = 0; j < kBH; j++) {
-#pragma unroll
-                for (int i = 0; i < kBW; i++) {
-                    float val;
-                    if (fc.subh == 0) {
-                        const float* row = s_chroma + p * plane_elems + (y0 + j - cy0) * ccols;
-                        const int c = kx[i] - cx0;
-                        val = cr_pick(cpx[i], wx[i], row + c, 1);
-                    } else {
-                        const int r = ky[j] - cy0;
-                        const float* colbase;
-                        int pitch;
-                        if (both) {
-                            colbase = s_mid + p * crows * kFrontTileW + (x0 + i - tile_x * kFrontTileW);
-                            pitch = kFrontTileW;
-                        } else {
-                            colbase = s_chroma + p * plane_elems + (x0 + i - cx0);
-                            pitch = ccols;
-                        }
-                        val = cr_pick(cpy[j], wy[j], colbase + r * pitch, pitch);
-                    }
-                    P[1 + p][j][i] = val;
-                }
+        if (fc.subw > 1) {
+            // 4:1:x: segment parameters of the thread's columns and rows
+            int kx[kBW], ky[kBH];
+            CrWeights wx[kBW], wy[kBH];
+            bool cpx[kBW], cpy[kBH];
+    #pragma unroll
+            for (int i = 0; i < kBW; i++) {
+                float tt = 0.0f;
+                if (fc.subw) seg_of(x0 + i, fc.subw, fc.h_cosited != 0, kx[i], tt, cpx[i]);
+                else { kx[i] = x0 + i; cpx[i] = true; }
+                wx[i] = cr_weights(tt);
             }
+    #pragma unroll
+            for (int j = 0; j < kBH; j++) {
+                float tt = 0.0f;
+                if (fc.subh) seg_of(y0 + j, fc.subh, fc.v_cosited != 0, ky[j], tt, cpy[j]);
+                else { ky[j] = y0 + j; cpy[j] = true; }
+                wy[j] = cr_weights(tt);
+            }
+    #pragma unroll
+            for (int p = 0; p < 2; p++) {
+    #pragma unroll
+                for (int j = 0; j < kBH; j++) {
+    #pragma unroll
+                    for (int i = 0; i < kBW; i++) {
+                        float val;
+                        if (fc.subh == 0) {
+                            const float* row = s_chroma + p * plane_elems + (y0 + j - cy0) * ccols;
+                            const int c = kx[i] - cx0;
+                            val = cr_pick(cpx[i], wx[i], row + c, 1);
+                        } else {
+                            const int r = ky[j] - cy0;
+                            const float* colbase;
+                            int pitch;
+                            if (both) {
+                                colbase = s_mid + p * crows * kFrontTileW + (x0 + i - tile_x * kFrontTileW);
+                                pitch = kFrontTileW;
+                            } else {
+                                colbase = s_chroma + p * plane_elems + (x0 + i - cx0);
+                                pitch = ccols;
+                            }
+                            val = cr_pick(cpy[j], wy[j], colbase + r * pitch, pitch);
+                        }
+                        P[1 + p][j][i] = val;
+                    }
+                }
+        }
         }
     }
 
@@ -614,7 +742,7 @@ size_t frontend_lds_bytes(const FrontColor& fc)
     if (fc.subw || fc.subh) {
         const size_t ccols = (kFrontTileW >> fc.subw) + 4, crows = (kFrontTileH >> fc.subh) + 4;
         chroma = 2 * ccols * crows;
-        if (fc.subw && fc.subh) chroma += 2 * crows * kFrontTileW;
+        if (fc.subw && fc.subh) chroma += 2 * crows * (fc.subw <= 1 ? kMidPitch : kFrontTileW);
     }
     const size_t levels = 3 * 16 * 33 + 3 * 8 * 17 + 3 * 4 * 9;
     return sizeof(float) * (chroma > levels ? chroma : levels);

@@ -81,6 +81,12 @@ struct FrontColor {
     float range12[2];      // v * mul + add of planes 1 and 2
     float pre[9];          // row-major
     float post[9];         // row-major
+    // Catmull-Rom weights of the two output phases (X = 2m + phase) of a
+    // 2x-subsampled axis over the 5-sample window [m-2, m+2] of the chroma
+    // plane (the cosited copy phase is the weight 1): hw horizontal, vw
+    // vertical. Used when subw <= 1 and subh <= 1.
+    float hw[2][5];
+    float vw[2][5];
 };
 
 // Pyramid geometry of one pair (both images share it).
