@@ -284,6 +284,22 @@ VSHIP_INLINE void down_lds(const float* src, int cols, int rows, int lx, int ly,
 // chromaUpsample.hpp:165-202), into s_mid with row pitch kMidPitch.
 constexpr int kMidPitch = kFrontTileW + 2;
 
+// LDS floats of the chroma staging (both planes with halo, plus the
+// horizontally upsampled rows when both axes are subsampled) of one layout,
+// the largest over the supported layouts, and of one image's level 1.
+constexpr int chroma_lds_floats(int subw, int subh)
+{
+    return (subw == 0 && subh == 0)
+               ? 0
+               : 2 * ((kFrontTileW >> subw) + 4) * ((kFrontTileH >> subh) + 4) +
+                     ((subw && subh) ? 2 * ((kFrontTileH >> subh) + 4) * (subw <= 1 ? kMidPitch : kFrontTileW) : 0);
+}
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int kChromaLdsMax =
+    cmax(cmax(cmax(chroma_lds_floats(1, 1), chroma_lds_floats(1, 0)), cmax(chroma_lds_floats(2, 0), chroma_lds_floats(2, 1))),
+         chroma_lds_floats(0, 1));
+constexpr int kL1Floats = 3 * 16 * 33;
+
 VSHIP_INLINE float dot5(const float w[5], const float v[5])
 {
     return fmaf(w[0], v[0], fmaf(w[1], v[1], fmaf(w[2], v[2], fmaf(w[3], v[3], w[4] * v[4]))));
@@ -506,8 +522,41 @@ VSHIP_INLINE void to_linear(const FrontJob& job, int k, const FrontColor& fc, in
     }
 }
 
+// Cross-lane partners of the level-2..4 reduction.
+VSHIP_INLINE float lane_xor1(float v) { return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xf, 0xf, false)); }
+VSHIP_INLINE float lane_xor2(float v) { return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xf, 0xf, false)); }
+VSHIP_INLINE float lane_xor16(float v) { return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F)); }
+VSHIP_INLINE float lane_xor32(float v) { return __shfl_xor(v, 32, 64); }
+
+// Next pyramid level from 2x2 parents held by lane pairs (x partner via fx,
+// y partner via fy) with the edge clamp min(2x+1, w-1) of
+// downsample.hpp:13-16: a parent beyond the level's edge takes its inside
+// partner's value. Every lane of a 2x2 lane group ends with the mean, summed
+// in the reference's order (p00 + p10 + p01 + p11).
+template <typename FX, typename FY>
+VSHIP_INLINE void down_pairs(float v[3], bool x_out, bool y_out, FX fx, FY fy)
+{
+    // the lane group's base lane (even x, even y) ends with the mean of its
+    // 2x2 parents, summed in the reference's order p00 + p10 + p01 + p11;
+    // lanes beyond the edge first take their partner's value (the clamp)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        float a = v[c];
+        const float ax = fx(a);
+        a = x_out ? ax : a;
+        const float ay = fy(a);
+        a = y_out ? ay : a;
+        const float p10 = fy(a), p01 = fx(a), p11 = fx(p10);
+        float s = a;
+        s += p10;
+        s += p01;
+        s += p11;
+        v[c] = s * 0.25f;
+    }
+}
+
 VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc, const PairGeom& geo, int tile_x,
-                               int tile_y, float* lds, TileXyb& res)
+                               int tile_y, float* lds, float* lds_l1, TileXyb& res)
 {
     const int t = threadIdx.x;
     const int tx = t % kTX, ty = t / kTX;
@@ -546,97 +595,150 @@ VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc,
         res.y1[j] = q.y;
     }
 
-    // ---- levels 2, 3, 4 through LDS (32x16 -> 16x8 -> 8x4 -> 4x2 per tile)
-    float* s_l1 = lds;                       // [3][16][33]
-    float* s_l2 = s_l1 + 3 * 16 * 33;        // [3][8][17]
-    float* s_l3 = s_l2 + 3 * 8 * 17;         // [3][4][9]
-    __syncthreads();                         // chroma staging no longer needed
+    // ---- levels 2, 3, 4 (32x16 -> 16x8 -> 8x4 -> 4x2 per tile): level 1
+    // through LDS (its own region per image, so no barrier protects it from
+    // the next image), then waves 0 and 1 hold one level-2 pixel per lane and
+    // reduce levels 3 and 4 across lanes with the same edge clamp
+    float* s_l1 = lds_l1;  // [3][16][33]
 #pragma unroll
     for (int j = 0; j < kBH / 2; j++)
 #pragma unroll
         for (int c = 0; c < 3; c++) s_l1[(c * 16 + ty * (kBH / 2) + j) * 33 + tx] = L1[j][c];
     __syncthreads();
     if (t < 128) {
-        const int lx = t & 15, ly = t >> 4;
+        const int lx = t & 15, ly = t >> 4;  // level-2 pixel of the lane
         float L2[3];
         down_lds(s_l1, 32, 16, lx, ly, tile_x * 32 + 2 * lx, tile_y * 16 + 2 * ly, geo.w[1], geo.h[1], L2);
+        {
+            const XYB q = to_xyb(L2[0], L2[1], L2[2], false);
+            res.x2 = q.x;
+            res.y2 = q.y;
+        }
+        // level 3: x partner lane ^ 1, y partner lane ^ 16; level 4: lane ^ 2
+        // and lane ^ 32 (each wave holds four level-2 rows)
+        float L3[3] = {L2[0], L2[1], L2[2]};
+        down_pairs(L3, tile_x * 16 + lx >= geo.w[2], tile_y * 8 + ly >= geo.h[2], lane_xor1, lane_xor16);
+        float L4[3] = {L3[0], L3[1], L3[2]};
+        down_pairs(L4, tile_x * 8 + (lx >> 1) >= geo.w[3], tile_y * 4 + (ly >> 1) >= geo.h[3], lane_xor2,
+                   lane_xor32);
+        // gather the wave's 16 level-3 and 4 level-4 pixels into lanes 0..19
+        // for one XYB pass: level 3 from its base lane (even, even), level 4
+        // moved one lane right of its base lane (x % 4 == 1), so that no lane
+        // serves both
+        const int lane = t & 63;
+        const bool l4src = (lx & 3) == 1 && (ly & 3) == 0;
+        const int src = lane < 16 ? 32 * (lane >> 3) + 2 * (lane & 7) : (lane < 20 ? 4 * (lane - 16) + 1 : lane);
+        float G[3];
 #pragma unroll
-        for (int c = 0; c < 3; c++) s_l2[(c * 8 + ly) * 17 + lx] = L2[c];
-        const XYB q = to_xyb(L2[0], L2[1], L2[2], false);
-        res.x2 = q.x;
-        res.y2 = q.y;
+        for (int c = 0; c < 3; c++) {
+            const float l4m = lane_xor1(L4[c]);
+            G[c] = __shfl(l4src ? l4m : L3[c], src, 64);
+        }
+        const XYB q = to_xyb(G[0], G[1], G[2], true);
+        if (lane < 16) {
+            res.l3[0] = q.x; res.l3[1] = q.y; res.l3[2] = q.b;
+        } else {
+            res.l4[0] = q.x; res.l4[1] = q.y; res.l4[2] = q.b;
+        }
     }
-    __syncthreads();
-    if (t < 32) {
-        const int lx = t & 7, ly = t >> 3;
-        float L3[3];
-        down_lds(s_l2, 16, 8, lx, ly, tile_x * 16 + 2 * lx, tile_y * 8 + 2 * ly, geo.w[2], geo.h[2], L3);
-#pragma unroll
-        for (int c = 0; c < 3; c++) s_l3[(c * 4 + ly) * 9 + lx] = L3[c];
-        const XYB q = to_xyb(L3[0], L3[1], L3[2], true);
-        res.l3[0] = q.x; res.l3[1] = q.y; res.l3[2] = q.b;
-    }
-    __syncthreads();
-    if (t < 8) {
-        const int lx = t & 3, ly = t >> 2;
-        float L4[3];
-        down_lds(s_l3, 8, 4, lx, ly, tile_x * 8 + 2 * lx, tile_y * 4 + 2 * ly, geo.w[3], geo.h[3], L4);
-        const XYB q = to_xyb(L4[0], L4[1], L4[2], true);
-        res.l4[0] = q.x; res.l4[1] = q.y; res.l4[2] = q.b;
-    }
-    __syncthreads();  // LDS is reused by the next image
 }
 
 // Writes the tile's stored planes. emit(plane, element offset, k) writes
-// plane element `off` from field value k of the thread's TileXyb.
-template <typename Emit>
-VSHIP_INLINE void store_tile(const FrontJob& job, const PairGeom& geo, int tile_x, int tile_y, Emit emit)
+// plane element `off` (32-bit: an image buffer holds fewer than 2^31
+// elements) from field value k of the thread's TileXyb; emit2(plane, off, k)
+// writes fields k and k + 1 to elements off and off + 1 (level-0 row pairs:
+// plane offsets, row pitches and x0 are even, so the pair is aligned).
+template <typename Emit, typename Emit2>
+VSHIP_INLINE void store_tile(const FrontJob& job, const PairGeom& geo, int tile_x, int tile_y, Emit emit,
+                             Emit2 emit2)
 {
     const int t = threadIdx.x;
     const int tx = t % kTX, ty = t / kTX;
     const int W = job.width, H = job.height;
     const int x0 = tile_x * kFrontTileW + kBW * tx;
     const int y0 = tile_y * kFrontTileH + kBH * ty;
+    static_assert(kBW == 2, "level-0 row pairs");
 #pragma unroll
-    for (int j = 0; j < kBH; j++)
-#pragma unroll
-        for (int i = 0; i < kBW; i++)
-            if (y0 + j < H && x0 + i < W)
-                emit(1, geo.off[0][1] + (int64_t)(y0 + j) * geo.pitch[0] + x0 + i, kFieldY0 + j * kBW + i);
+    for (int j = 0; j < kBH; j++) {
+        if (y0 + j < H) {
+            const uint32_t o = (uint32_t)geo.off[0][1] + (uint32_t)(y0 + j) * (uint32_t)geo.pitch[0] + (uint32_t)x0;
+            if (x0 + 1 < W) {
+                emit2(1, o, kFieldY0 + j * kBW);
+            } else if (x0 < W) {
+                emit(1, o, kFieldY0 + j * kBW);
+            }
+        }
+    }
     const int x1 = tile_x * 32 + tx;
 #pragma unroll
     for (int j = 0; j < kBH / 2; j++) {
         const int y1 = tile_y * 16 + ty * (kBH / 2) + j;
         if (y1 < geo.h[1] && x1 < geo.w[1]) {
-            const int64_t o = (int64_t)y1 * geo.pitch[1] + x1;
-            emit(0, geo.off[1][0] + o, kFieldX1 + j);
-            emit(1, geo.off[1][1] + o, kFieldY1 + j);
+            const uint32_t o = (uint32_t)y1 * (uint32_t)geo.pitch[1] + (uint32_t)x1;
+            emit(0, (uint32_t)geo.off[1][0] + o, kFieldX1 + j);
+            emit(1, (uint32_t)geo.off[1][1] + o, kFieldY1 + j);
         }
     }
     if (t < 128) {
         const int x2 = tile_x * 16 + (t & 15), y2 = tile_y * 8 + (t >> 4);
         if (x2 < geo.w[2] && y2 < geo.h[2]) {
-            const int64_t o = (int64_t)y2 * geo.pitch[2] + x2;
-            emit(0, geo.off[2][0] + o, kFieldX2);
-            emit(1, geo.off[2][1] + o, kFieldY2);
+            const uint32_t o = (uint32_t)y2 * (uint32_t)geo.pitch[2] + (uint32_t)x2;
+            emit(0, (uint32_t)geo.off[2][0] + o, kFieldX2);
+            emit(1, (uint32_t)geo.off[2][1] + o, kFieldY2);
         }
     }
-    if (t < 32) {
-        const int x3 = tile_x * 8 + (t & 7), y3 = tile_y * 4 + (t >> 3);
-        if (x3 < geo.w[3] && y3 < geo.h[3]) {
-            const int64_t o = (int64_t)y3 * geo.pitch[3] + x3;
+    // levels 3 and 4: lanes 0..15 and 16..19 of waves 0 and 1 (wave w holds
+    // level-3 rows 2w, 2w + 1 and level-4 row w)
+    if (t < 128) {
+        const int lane = t & 63, w = t >> 6;
+        if (lane < 16) {
+            const int x3 = tile_x * 8 + (lane & 7), y3 = tile_y * 4 + 2 * w + (lane >> 3);
+            if (x3 < geo.w[3] && y3 < geo.h[3]) {
+                const uint32_t o = (uint32_t)y3 * (uint32_t)geo.pitch[3] + (uint32_t)x3;
 #pragma unroll
-            for (int c = 0; c < 3; c++) emit(c, geo.off[3][c] + o, kFieldL3 + c);
-        }
-    }
-    if (t < 8) {
-        const int x4 = tile_x * 4 + (t & 3), y4 = tile_y * 2 + (t >> 2);
-        if (x4 < geo.w[4] && y4 < geo.h[4]) {
-            const int64_t o = (int64_t)y4 * geo.pitch[4] + x4;
+                for (int c = 0; c < 3; c++) emit(c, (uint32_t)geo.off[3][c] + o, kFieldL3 + c);
+            }
+        } else if (lane < 20) {
+            const int x4 = tile_x * 4 + (lane - 16), y4 = tile_y * 2 + w;
+            if (x4 < geo.w[4] && y4 < geo.h[4]) {
+                const uint32_t o = (uint32_t)y4 * (uint32_t)geo.pitch[4] + (uint32_t)x4;
 #pragma unroll
-            for (int c = 0; c < 3; c++) emit(c, geo.off[4][c] + o, kFieldL4 + c);
+                for (int c = 0; c < 3; c++) emit(c, (uint32_t)geo.off[4][c] + o, kFieldL4 + c);
+            }
         }
     }
+}
+
+// Global-memory stores of one or two consecutive plane elements.
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2_t __attribute__((ext_vector_type(2)));
+
+template <typename StoreT>
+VSHIP_INLINE void gstore1(void* base, uint32_t off, float v);
+template <>
+VSHIP_INLINE void gstore1<float>(void* base, uint32_t off, float v)
+{
+    ((__attribute__((address_space(1))) float*)base)[off] = v;
+}
+template <>
+VSHIP_INLINE void gstore1<__half>(void* base, uint32_t off, float v)
+{
+    ((__attribute__((address_space(1))) _Float16*)base)[off] = (_Float16)v;
+}
+
+template <typename StoreT>
+VSHIP_INLINE void gstore2(void* base, uint32_t off, float v0, float v1);
+template <>
+VSHIP_INLINE void gstore2<float>(void* base, uint32_t off, float v0, float v1)
+{
+    *(__attribute__((address_space(1))) f32x2_t*)((__attribute__((address_space(1))) float*)base + off) =
+        f32x2_t{v0, v1};
+}
+template <>
+VSHIP_INLINE void gstore2<__half>(void* base, uint32_t off, float v0, float v1)
+{
+    *(__attribute__((address_space(1))) f16x2_t*)((__attribute__((address_space(1))) _Float16*)base + off) =
+        f16x2_t{(_Float16)v0, (_Float16)v1};
 }
 
 // FP32 layout: each image is written as soon as it is computed. FP16 layout:
@@ -653,6 +755,7 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
     constexpr bool kHalf = sizeof(StoreT) == 2;
     float* stash = lds;
     float* work = lds + (kHalf ? kFrontStashFloats : 0);
+    float* l1_area = work + kChromaLdsMax;  // [2 images][kL1Floats]
     const int t = threadIdx.x;
     const int total = counts[kCountFrontTiles];
     const int njobs = counts[kCountFrontJobs];
@@ -677,23 +780,32 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
 #pragma unroll 1
             for (int k = 0; k < 2; k++) {
                 TileXyb r;
-                compute_tile(job, k, colors[job.color[k]], geo, tile_x, tile_y, work, r);
-                StoreT* out = static_cast<StoreT*>(job.out[k]);
-                store_tile(job, geo, tile_x, tile_y, [&](int, int64_t off, int f) { out[off] = field(r, f); });
+                compute_tile(job, k, colors[job.color[k]], geo, tile_x, tile_y, work, l1_area + k * kL1Floats, r);
+                void* out = job.out[k];
+                store_tile(
+                    job, geo, tile_x, tile_y, [&](int, uint32_t off, int f) { gstore1<StoreT>(out, off, field(r, f)); },
+                    [&](int, uint32_t off, int f) { gstore2<StoreT>(out, off, field(r, f), field(r, f + 1)); });
             }
         } else {
             TileXyb a, b;
-            compute_tile(job, 0, colors[job.color[0]], geo, tile_x, tile_y, work, a);
+            compute_tile(job, 0, colors[job.color[0]], geo, tile_x, tile_y, work, l1_area, a);
 #pragma unroll
             for (int f = 0; f < kFields; f++) stash[f * kFrontThreads + t] = field(a, f);
-            compute_tile(job, 1, colors[job.color[1]], geo, tile_x, tile_y, work, b);
-            StoreT* o0 = static_cast<StoreT*>(job.out[0]);
-            StoreT* o1 = static_cast<StoreT*>(job.out[1]);
-            store_tile(job, geo, tile_x, tile_y, [&](int plane, int64_t off, int f) {
-                const float va = stash[f * kFrontThreads + t];
-                o0[off] = store_cast<StoreT>(va - kHalfOffset[plane]);
-                o1[off] = store_cast<StoreT>(field(b, f) - va);
-            });
+            compute_tile(job, 1, colors[job.color[1]], geo, tile_x, tile_y, work, l1_area + kL1Floats, b);
+            void* o0 = job.out[0];
+            void* o1 = job.out[1];
+            store_tile(
+                job, geo, tile_x, tile_y,
+                [&](int plane, uint32_t off, int f) {
+                    const float va = stash[f * kFrontThreads + t];
+                    gstore1<StoreT>(o0, off, va - kHalfOffset[plane]);
+                    gstore1<StoreT>(o1, off, field(b, f) - va);
+                },
+                [&](int plane, uint32_t off, int f) {
+                    const float va0 = stash[f * kFrontThreads + t], va1 = stash[(f + 1) * kFrontThreads + t];
+                    gstore2<StoreT>(o0, off, va0 - kHalfOffset[plane], va1 - kHalfOffset[plane]);
+                    gstore2<StoreT>(o1, off, field(b, f) - va0, field(b, f + 1) - va1);
+                });
         }
         __syncthreads();  // LDS is reused by the next item
     }
@@ -768,8 +880,11 @@ hipError_t launch_frontend(const FrontJob* jobs, const int* counts, int grid, co
 {
     if (grid <= 0) return hipSuccess;
     grid = (grid + kXcds - 1) / kXcds * kXcds;  // the item mapping needs whole XCD rounds
-    if (sizeof(StoreT) == 2) lds_bytes += kFrontStashFloats * sizeof(float);
-    hipLaunchKernelGGL(frontend_kernel<StoreT>, dim3(grid), dim3(kFrontThreads), lds_bytes, stream,
+    // chroma staging (largest layout), level 1 of each image, FP16 stash
+    (void)lds_bytes;
+    const size_t bytes = sizeof(float) * ((size_t)kChromaLdsMax + 2 * kL1Floats +
+                                          (sizeof(StoreT) == 2 ? (size_t)kFrontStashFloats : 0));
+    hipLaunchKernelGGL(frontend_kernel<StoreT>, dim3(grid), dim3(kFrontThreads), bytes, stream,
                        jobs, counts, colors, geoms);
     return hipGetLastError();
 }
