@@ -384,7 +384,6 @@ Ssimu2Handler::Ssimu2Handler(const Vship_Colorspace_t& src, const Vship_Colorspa
     };
     const size_t elems = (size_t)layout_.geom.image_elems;
     for (int i = 0; i < 2; i++) dmalloc(&xyb_[i], elems * sizeof(float));
-    dmalloc((void**)&d_front_jobs_, sizeof(FrontJob));
     dmalloc((void**)&d_colors_, 3 * sizeof(FrontColor));
     dmalloc((void**)&d_geom_, sizeof(PairGeom));
     dmalloc((void**)&d_maps_, layout_.maps.size() * sizeof(MapJob));
@@ -478,7 +477,6 @@ Ssimu2Handler::~Ssimu2Handler()
     }
     (void)hipFree(d_lin_jobs_);
     if (h_lin_jobs_) (void)hipHostFree(h_lin_jobs_);
-    (void)hipFree(d_front_jobs_);
     (void)hipFree(d_colors_);
     (void)hipFree(d_geom_);
     (void)hipFree(d_maps_);
@@ -529,12 +527,13 @@ void Ssimu2Handler::launch_all()
             if (th != H) VSHIP_HIP_CHECK(launch_resize_v(rsz_[i], cur, tw, H, th, stream_));
         }
     }
-    VSHIP_HIP_CHECK(hipMemcpyAsync(d_front_jobs_, h_front_jobs_, sizeof(FrontJob), hipMemcpyHostToDevice, stream_));
-    VSHIP_HIP_CHECK(launch_frontend<float>(d_front_jobs_, d_counts_, layout_.front_tiles_per_image, d_colors_, d_geom_,
-                                           front_lds_, stream_));
+    // the job (plane pointers of this call) travels as a kernel argument
+    VSHIP_HIP_CHECK(launch_frontend_single<float>(*h_front_jobs_, layout_.front_tiles_per_image, d_colors_, d_geom_,
+                                                  stream_));
     VSHIP_HIP_CHECK(launch_blur<float>(layout_.blur_config, d_maps_, d_counts_, blur_grid_for(layout_.blur_tiles),
                                        taps_, d_partials_, stream_));
-    VSHIP_HIP_CHECK(launch_score(d_score_jobs_, d_counts_, 1, d_maps_, d_partials_, d_features_, d_score_, stream_));
+    // the score goes straight to the pinned host word the caller reads
+    VSHIP_HIP_CHECK(launch_score(d_score_jobs_, d_counts_, 1, d_maps_, d_partials_, d_features_, h_score_, stream_));
 }
 
 void Ssimu2Handler::run(const uint8_t* const p1[3], const uint8_t* const p2[3], const int64_t ls1[3],
@@ -553,7 +552,6 @@ void Ssimu2Handler::run(const uint8_t* const p1[3], const uint8_t* const p2[3], 
         }
     }
     launch_all();
-    VSHIP_HIP_CHECK(hipMemcpyAsync(h_score_, d_score_, sizeof(double), hipMemcpyDeviceToHost, stream_));
     VSHIP_HIP_CHECK(hipStreamSynchronize(stream_));
     have_run_ = true;
 }
@@ -622,8 +620,8 @@ void Ssimu2Handler::profile(int iters, double* ms_front, double* ms_blur, double
     double acc[3] = {0, 0, 0};
     for (int it = 0; it < iters; it++) {
         VSHIP_HIP_CHECK(hipEventRecord(ev[0], stream_));
-        VSHIP_HIP_CHECK(launch_frontend<float>(d_front_jobs_, d_counts_, layout_.front_tiles_per_image, d_colors_,
-                                               d_geom_, front_lds_, stream_));
+        VSHIP_HIP_CHECK(launch_frontend_single<float>(*h_front_jobs_, layout_.front_tiles_per_image, d_colors_,
+                                                      d_geom_, stream_));
         VSHIP_HIP_CHECK(hipEventRecord(ev[1], stream_));
         VSHIP_HIP_CHECK(launch_blur<float>(layout_.blur_config, d_maps_, d_counts_, blur_grid_for(layout_.blur_tiles),
                                            taps_, d_partials_, stream_));

@@ -93,7 +93,6 @@ private:
     size_t front_lds_ = 0;
 
     void* xyb_[2] = {nullptr, nullptr};       // float planes per image
-    FrontJob* d_front_jobs_ = nullptr;
     FrontColor* d_colors_ = nullptr;
     PairGeom* d_geom_ = nullptr;
     MapJob* d_maps_ = nullptr;

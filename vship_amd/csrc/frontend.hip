@@ -741,27 +741,65 @@ VSHIP_INLINE void gstore2<__half>(void* base, uint32_t off, float v0, float v1)
         f16x2_t{(_Float16)v0, (_Float16)v1};
 }
 
-// FP32 layout: each image is written as soon as it is computed. FP16 layout:
-// the reference plane holds fp16(a - offset) and the distorted plane
-// fp16(b - a) of the FP32 values, so the reference image's values wait in an
-// LDS stash (field-major, one float per thread and field) while the
-// distorted image is computed.
+// One tile of both images of a pair. FP32 layout: each image is written as
+// soon as it is computed. FP16 layout: the reference plane holds
+// fp16(a - offset) and the distorted plane fp16(b - a) of the FP32 values, so
+// the reference image's values wait in an LDS stash (field-major, one float
+// per thread and field) while the distorted image is computed.
+template <typename StoreT>
+VSHIP_INLINE void frontend_tile(const FrontJob& job, const FrontColor* __restrict__ colors, const PairGeom& geo,
+                                int tile_x, int tile_y, float* lds)
+{
+    constexpr bool kHalf = sizeof(StoreT) == 2;
+    float* stash = lds;
+    float* work = lds + (kHalf ? kFrontStashFloats : 0);
+    float* l1_area = work + kChromaLdsMax;  // [2 images][kL1Floats]
+    const int t = threadIdx.x;
+    if constexpr (!kHalf) {
+#pragma unroll 1
+        for (int k = 0; k < 2; k++) {
+            TileXyb r;
+            compute_tile(job, k, colors[job.color[k]], geo, tile_x, tile_y, work, l1_area + k * kL1Floats, r);
+            void* out = job.out[k];
+            store_tile(
+                job, geo, tile_x, tile_y, [&](int, uint32_t off, int f) { gstore1<StoreT>(out, off, field(r, f)); },
+                [&](int, uint32_t off, int f) { gstore2<StoreT>(out, off, field(r, f), field(r, f + 1)); });
+        }
+    } else {
+        TileXyb a, b;
+        compute_tile(job, 0, colors[job.color[0]], geo, tile_x, tile_y, work, l1_area, a);
+#pragma unroll
+        for (int f = 0; f < kFields; f++) stash[f * kFrontThreads + t] = field(a, f);
+        compute_tile(job, 1, colors[job.color[1]], geo, tile_x, tile_y, work, l1_area + kL1Floats, b);
+        void* o0 = job.out[0];
+        void* o1 = job.out[1];
+        store_tile(
+            job, geo, tile_x, tile_y,
+            [&](int plane, uint32_t off, int f) {
+                const float va = stash[f * kFrontThreads + t];
+                gstore1<StoreT>(o0, off, va - kHalfOffset[plane]);
+                gstore1<StoreT>(o1, off, field(b, f) - va);
+            },
+            [&](int plane, uint32_t off, int f) {
+                const float va0 = stash[f * kFrontThreads + t], va1 = stash[(f + 1) * kFrontThreads + t];
+                gstore2<StoreT>(o0, off, va0 - kHalfOffset[plane], va1 - kHalfOffset[plane]);
+                gstore2<StoreT>(o1, off, field(b, f) - va0, field(b, f + 1) - va1);
+            });
+    }
+}
+
+// Persistent front-end over the job table (batches). Workgroups go
+// round-robin over the XCDs: each XCD takes a contiguous range of tiles, so
+// horizontally adjacent tiles, which share the 128-byte lines of their input
+// rows, are fetched through the same L2.
 template <typename StoreT>
 __global__ __launch_bounds__(kFrontThreads, 4)
 void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ counts,
                      const FrontColor* __restrict__ colors, const PairGeom* __restrict__ geoms)
 {
     extern __shared__ float lds[];
-    constexpr bool kHalf = sizeof(StoreT) == 2;
-    float* stash = lds;
-    float* work = lds + (kHalf ? kFrontStashFloats : 0);
-    float* l1_area = work + kChromaLdsMax;  // [2 images][kL1Floats]
-    const int t = threadIdx.x;
     const int total = counts[kCountFrontTiles];
     const int njobs = counts[kCountFrontJobs];
-    // workgroups go round-robin over the XCDs: each XCD takes a contiguous
-    // range of tiles, so horizontally adjacent tiles, which share the 128-byte
-    // lines of their input rows, are fetched through the same L2
     const int per = (total + kXcds - 1) / kXcds;
     const int xcd = blockIdx.x % kXcds;
     const int end = min((xcd + 1) * per, total);
@@ -773,42 +811,25 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
             if (jobs[mid].tile_start <= item) lo = mid; else hi = mid - 1;
         }
         const FrontJob& job = jobs[lo];
-        const PairGeom& geo = geoms[job.pair];
         const int local = item - job.tile_start;
-        const int tile_x = local % job.tiles_x, tile_y = local / job.tiles_x;
-        if constexpr (!kHalf) {
-#pragma unroll 1
-            for (int k = 0; k < 2; k++) {
-                TileXyb r;
-                compute_tile(job, k, colors[job.color[k]], geo, tile_x, tile_y, work, l1_area + k * kL1Floats, r);
-                void* out = job.out[k];
-                store_tile(
-                    job, geo, tile_x, tile_y, [&](int, uint32_t off, int f) { gstore1<StoreT>(out, off, field(r, f)); },
-                    [&](int, uint32_t off, int f) { gstore2<StoreT>(out, off, field(r, f), field(r, f + 1)); });
-            }
-        } else {
-            TileXyb a, b;
-            compute_tile(job, 0, colors[job.color[0]], geo, tile_x, tile_y, work, l1_area, a);
-#pragma unroll
-            for (int f = 0; f < kFields; f++) stash[f * kFrontThreads + t] = field(a, f);
-            compute_tile(job, 1, colors[job.color[1]], geo, tile_x, tile_y, work, l1_area + kL1Floats, b);
-            void* o0 = job.out[0];
-            void* o1 = job.out[1];
-            store_tile(
-                job, geo, tile_x, tile_y,
-                [&](int plane, uint32_t off, int f) {
-                    const float va = stash[f * kFrontThreads + t];
-                    gstore1<StoreT>(o0, off, va - kHalfOffset[plane]);
-                    gstore1<StoreT>(o1, off, field(b, f) - va);
-                },
-                [&](int plane, uint32_t off, int f) {
-                    const float va0 = stash[f * kFrontThreads + t], va1 = stash[(f + 1) * kFrontThreads + t];
-                    gstore2<StoreT>(o0, off, va0 - kHalfOffset[plane], va1 - kHalfOffset[plane]);
-                    gstore2<StoreT>(o1, off, field(b, f) - va0, field(b, f + 1) - va1);
-                });
-        }
+        frontend_tile<StoreT>(job, colors, geoms[job.pair], local % job.tiles_x, local / job.tiles_x, lds);
         __syncthreads();  // LDS is reused by the next item
     }
+}
+
+// One pair whose job travels as a kernel argument (the single-pair handler:
+// no upload of the job before each call). One workgroup per tile, in the
+// same XCD-contiguous order.
+template <typename StoreT>
+__global__ __launch_bounds__(kFrontThreads, 4)
+void frontend_single_kernel(FrontJob job, int tiles, const FrontColor* __restrict__ colors,
+                            const PairGeom* __restrict__ geoms)
+{
+    extern __shared__ float lds[];
+    const int per = (tiles + kXcds - 1) / kXcds;
+    const int item = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+    if (item >= tiles || (int)(blockIdx.x / kXcds) >= per) return;
+    frontend_tile<StoreT>(job, colors, geoms[job.pair], item % job.tiles_x, item / job.tiles_x, lds);
 }
 
 // Linear RGB of image 0 of one job, stored as three W x H planes with row
@@ -888,6 +909,21 @@ hipError_t launch_frontend(const FrontJob* jobs, const int* counts, int grid, co
                        jobs, counts, colors, geoms);
     return hipGetLastError();
 }
+
+template <typename StoreT>
+hipError_t launch_frontend_single(const FrontJob& job, int tiles, const FrontColor* colors, const PairGeom* geoms,
+                                  hipStream_t stream)
+{
+    if (tiles <= 0) return hipSuccess;
+    const int grid = (tiles + kXcds - 1) / kXcds * kXcds;
+    const size_t bytes = sizeof(float) * ((size_t)kChromaLdsMax + 2 * kL1Floats +
+                                          (sizeof(StoreT) == 2 ? (size_t)kFrontStashFloats : 0));
+    hipLaunchKernelGGL(frontend_single_kernel<StoreT>, dim3(grid), dim3(kFrontThreads), bytes, stream, job, tiles,
+                       colors, geoms);
+    return hipGetLastError();
+}
+
+template hipError_t launch_frontend_single<float>(const FrontJob&, int, const FrontColor*, const PairGeom*, hipStream_t);
 
 template hipError_t launch_frontend<float>(const FrontJob*, const int*, int, const FrontColor*, const PairGeom*, size_t, hipStream_t);
 template hipError_t launch_frontend<__half>(const FrontJob*, const int*, int, const FrontColor*, const PairGeom*, size_t, hipStream_t);

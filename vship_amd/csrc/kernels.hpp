@@ -16,6 +16,12 @@ template <typename StoreT>
 hipError_t launch_frontend(const FrontJob* jobs, const int* counts, int grid, const FrontColor* colors,
                            const PairGeom* geoms, size_t lds_bytes, hipStream_t stream);
 
+// One pair's tiles (tiles = the job's 64x32 tile count) with the job passed
+// by value as a kernel argument, one workgroup per tile.
+template <typename StoreT>
+hipError_t launch_frontend_single(const FrontJob& job, int tiles, const FrontColor* colors, const PairGeom* geoms,
+                                  hipStream_t stream);
+
 // Linear RGB of image 0 of `job` (tiles = job's 64x32 tile count) into three
 // planes of job.width x job.height floats, row pitch `pitch` elements.
 hipError_t launch_linear_rgb(const FrontJob* job, int tiles, const FrontColor* colors, float* out, int64_t pitch,
