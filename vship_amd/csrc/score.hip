@@ -4,10 +4,10 @@
 // of the pair: the wave sums the per-tile float partials in double, divides by the
 // scale's pixel count (L1 = mean, L4 = mean(x^4)^(1/4), reference
 // src/ssimu2/score.hpp:315-320,422-438), then forms the weighted sum of the
-// 108 features and the polynomial remap (score.hpp:442-460) in double. The
-// reference accumulates the weighted sum in float and returns a float; this
-// kernel keeps every step in double. Summation order is fixed, so results are
-// bit-reproducible run to run.
+// 108 features (one wave, fixed-order reduction) and the polynomial remap
+// (score.hpp:442-460) in double. The reference accumulates the weighted sum in
+// float and returns a float; this kernel keeps every step in double.
+// Summation order is fixed, so results are bit-reproducible run to run.
 #include "ssimu2_common.hpp"
 #include "weights.hpp"
 #include "kernels.hpp"
@@ -65,12 +65,17 @@ void score_kernel(const ScoreJob* __restrict__ pairs, const int* __restrict__ co
             }
         }
         __syncthreads();
-        if (t == 0) {
-            double s = 0.0;
-            for (int i = 0; i < 108; i++) s = fma(d_weights[i], feat[i], s);
-            s *= 0.9562382616834844;
-            s = (6.248496625763138e-5 * s * s) * s + 2.326765642916932 * s - 0.020884521182843837 * s * s;
-            scores[pi] = (s > 0.0) ? pow(s, 0.6276336467831387) * -10.0 + 100.0 : 100.0;
+        if (wave == 0) {
+            // weighted sum of the 108 features: lane k takes features k and
+            // k + 64, then a fixed-order wave reduction (all in double)
+            double s = lane < 108 ? d_weights[lane] * feat[lane] : 0.0;
+            if (lane + 64 < 108) s = fma(d_weights[lane + 64], feat[lane + 64], s);
+            s = wave_sum_d(s);
+            if (lane == 0) {
+                s *= 0.9562382616834844;
+                s = (6.248496625763138e-5 * s * s) * s + 2.326765642916932 * s - 0.020884521182843837 * s * s;
+                scores[pi] = (s > 0.0) ? pow(s, 0.6276336467831387) * -10.0 + 100.0 : 100.0;
+            }
         }
         if (features)
             for (int i = t; i < 108; i += kScoreThreads) features[(int64_t)pi * 108 + i] = feat[i];
