@@ -187,6 +187,41 @@ __device__ inline Toeplitz make_toeplitz(const BlurTaps& taps, int lane)
     return T;
 }
 
+// The fragments of every lane, shared by the workgroup's waves through LDS
+// (structure of arrays, one 16-byte vector per lane and fragment pair, so the
+// reads are conflict-free); loaded right before the products that use them,
+// which keeps 16 registers free across the item.
+struct ToeplitzLds {
+    half8 g_hi[64], g_lo[64];
+    half8 t01[64];  // t0_hi, t0_lo
+    half8 t1[64];   // t1_hi, t1_lo
+};
+
+__device__ inline void toeplitz_to_lds(const Toeplitz& T, int lane, ToeplitzLds* L)
+{
+    L->g_hi[lane] = T.g_hi;
+    L->g_lo[lane] = T.g_lo;
+    half8 a, b;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        a[j] = T.t0_hi[j];
+        a[4 + j] = T.t0_lo[j];
+        b[j] = T.t1_hi[j];
+        b[4 + j] = T.t1_lo[j];
+    }
+    L->t01[lane] = a;
+    L->t1[lane] = b;
+}
+
+__device__ inline void split_half8(const half8& v, half4& lo4, half4& hi4)
+{
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        lo4[j] = v[j];
+        hi4[j] = v[4 + j];
+    }
+}
+
 template <typename StoreT>
 struct StageRegs;
 
@@ -284,7 +319,7 @@ __device__ inline void stage_store(const StageRegs<StoreT>& r, const StripGeom& 
 
 template <typename StoreT, int NQ>
 __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, const BlurTaps& taps,
-                          const Toeplitz& T, float* sA, float* sB, float* red, float* __restrict__ partial)
+                          const ToeplitzLds* TL, float* sA, float* sB, float* red, float* __restrict__ partial)
 {
     const int t = threadIdx.x;
     const int wave = t >> 6, lane = t & 63;
@@ -351,6 +386,7 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
         // ---- per quantity: operand hi/lo, horizontal product, hi/lo of the
         // results (the vertical B operand)
         half4 bh[NQ], bl[NQ];
+        const half8 g_hi = TL->g_hi[lane], g_lo = TL->g_lo[lane];
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
             half8 ah, al;
@@ -373,7 +409,7 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
                 split2(v0, v1, q < 2 ? s1 : (q == 2 ? s2 : s2h), h0, h1, l0, l1);
                 ah[j] = h0; ah[j + 1] = h1; al[j] = l0; al[j + 1] = l1;
             }
-            const floatx4 hq = mma32(ah, al, T.g_hi, T.g_lo);
+            const floatx4 hq = mma32(ah, al, g_hi, g_lo);
             const float hv[4] = {hq[0], hq[1], hq[2], hq[3]};
             split4(hv, bh[q], bl[q]);
         }
@@ -381,10 +417,12 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
         // ---- output block kb - 1 is complete: T1 part, unscale, maps
         if (kb >= 1) {
             const int rbk = kb - 1;
+            half4 t1_hi, t1_lo;
+            split_half8(TL->t1[lane], t1_hi, t1_lo);
             floatx4 v[NQ];
 #pragma unroll
             for (int q = 0; q < NQ; q++) {
-                const floatx4 t1 = mma16(T.t1_hi, T.t1_lo, bh[q], bl[q]);
+                const floatx4 t1 = mma16(t1_hi, t1_lo, bh[q], bl[q]);
                 const float up = q < 2 ? prev_u1 : prev_u2, uc = q < 2 ? u1 : u2;
 #pragma unroll
                 for (int i = 0; i < 4; i++) v[q][i] = prev[q][i] * up + t1[i] * uc;
@@ -427,8 +465,10 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
         }
         // ---- T0 part of output block kb
         if (kb < nrb) {
+            half4 t0_hi, t0_lo;
+            split_half8(TL->t01[lane], t0_hi, t0_lo);
 #pragma unroll
-            for (int q = 0; q < NQ; q++) prev[q] = mma16(T.t0_hi, T.t0_lo, bh[q], bl[q]);
+            for (int q = 0; q < NQ; q++) prev[q] = mma16(t0_hi, t0_lo, bh[q], bl[q]);
             prev_u1 = u1;
             prev_u2 = u2;
         }
@@ -455,13 +495,16 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
 }
 
 template <typename StoreT>
-__global__ __launch_bounds__(kBlurThreads)
+// 4 waves per SIMD: 128 registers per lane, 36.4 KB of LDS per workgroup.
+__global__ __launch_bounds__(kBlurThreads, 4)
 void blur_strip_kernel(const MapJob* __restrict__ jobs, const int* __restrict__ counts, BlurTaps taps, int segb,
                        float* __restrict__ partials)
 {
     __shared__ __attribute__((aligned(16))) float ring[2 * kSlots * kSlotFloats];
     __shared__ float red[32];
-    const Toeplitz T = make_toeplitz(taps, threadIdx.x & 63);
+    __shared__ ToeplitzLds toe;
+    if (threadIdx.x < 64) toeplitz_to_lds(make_toeplitz(taps, threadIdx.x), threadIdx.x, &toe);
+    __syncthreads();
     float* sA = ring;
     float* sB = ring + kSlots * kSlotFloats;
     const int total = counts[kCountBlurTiles];
@@ -482,10 +525,10 @@ void blur_strip_kernel(const MapJob* __restrict__ jobs, const int* __restrict__ 
         const int tile_y = local / job.tiles_x;
         __syncthreads();  // LDS of the previous item is free
         if (job.flags & kMapSsim)
-            blur_item<StoreT, 4>(job, tile_x, tile_y, segb, taps, T, sA, sB, red,
+            blur_item<StoreT, 4>(job, tile_x, tile_y, segb, taps, &toe, sA, sB, red,
                                  partials + (int64_t)item * kPartialStride);
         else
-            blur_item<StoreT, 2>(job, tile_x, tile_y, segb, taps, T, sA, sB, red,
+            blur_item<StoreT, 2>(job, tile_x, tile_y, segb, taps, &toe, sA, sB, red,
                                  partials + (int64_t)item * kPartialStride);
     }
 }

@@ -24,11 +24,11 @@ constexpr float kC2 = 0.0009f;    // SSIM constant (score.hpp:272-273)
 // threads, each owning a 2x2 block.
 constexpr int kXcds = 8;  // MI355X: workgroups are dispatched round-robin over 8 XCDs, each with its own L2
 // Blur tile configs (blur_tile_shape): the single-pair handler uses 64x128
-// strips (7) and the batch 64x256 strips (8) with 3 workgroups per CU, the
+// strips (7) and the batch 64x256 strips (8) with 4 workgroups per CU, the
 // fastest measured on MI355X for one 1080p pair and for 240 of them.
 constexpr int kDefaultBlurConfig = 7;
 constexpr int kDefaultBatchBlurConfig = 8;
-constexpr int kDefaultBatchBlurWaves = 3;
+constexpr int kDefaultBatchBlurWaves = 4;
 constexpr int kFrontTileW = 64;
 constexpr int kFrontTileH = 32;
 constexpr int kFrontThreads = 512;
