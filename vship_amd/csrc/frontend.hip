@@ -98,22 +98,32 @@ typedef const __attribute__((address_space(1))) uint32_t GU32;
 // the LDS wait counter). Samples of planes whose base and row stride are
 // multiples of the sample size are read with one load, others byte by byte.
 template <int KIND>
-VSHIP_INLINE float load_kind(const uint8_t* p8, uint32_t mask, bool aligned)
+VSHIP_INLINE uint32_t raw_kind(const uint8_t* p8, bool aligned)
 {
     GByte* p = (GByte*)p8;
     if constexpr (KIND == kSampleU8) {
-        return (float)p[0];
-    } else if constexpr (KIND == kSampleU16) {
-        const uint32_t v = aligned ? (uint32_t)*(GU16*)p : ((uint32_t)p[0] | ((uint32_t)p[1] << 8));
-        return (float)(v & mask);
-    } else if constexpr (KIND == kSampleHalf) {
-        const uint32_t v = aligned ? (uint32_t)*(GU16*)p : ((uint32_t)p[0] | ((uint32_t)p[1] << 8));
-        return __half2float(__ushort_as_half((uint16_t)v));
+        return (uint32_t)p[0];
+    } else if constexpr (KIND == kSampleU16 || KIND == kSampleHalf) {
+        return aligned ? (uint32_t)*(GU16*)p : ((uint32_t)p[0] | ((uint32_t)p[1] << 8));
     } else {
-        return __uint_as_float(aligned ? *(GU32*)p
-                                       : ((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) |
-                                          ((uint32_t)p[3] << 24)));
+        return aligned ? *(GU32*)p
+                       : ((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24));
     }
+}
+
+template <int KIND>
+VSHIP_INLINE float convert_kind(uint32_t v, uint32_t mask)
+{
+    if constexpr (KIND == kSampleU8) return (float)v;
+    else if constexpr (KIND == kSampleU16) return (float)(v & mask);
+    else if constexpr (KIND == kSampleHalf) return __half2float(__ushort_as_half((uint16_t)v));
+    else return __uint_as_float(v);
+}
+
+template <int KIND>
+VSHIP_INLINE float load_kind(const uint8_t* p8, uint32_t mask, bool aligned)
+{
+    return convert_kind<KIND>(raw_kind<KIND>(p8, aligned), mask);
 }
 
 template <int KIND>
@@ -191,6 +201,128 @@ VSHIP_INLINE void stage_chroma(const uint8_t* const* plane, const int64_t* pstri
         }
 }
 
+// Raw samples of one image of a tile, loaded ahead of their use (the batched
+// front-end issues an image's loads while the previous image is computed):
+// r[0..3] the thread's 2x2 luma block; then either the 2x2 blocks of planes 1
+// and 2 (no chroma subsampling) or the thread's chroma staging samples
+// (r[4 + 3p + it] = item it of chroma plane p).
+constexpr int kRawWords = 12;
+struct RawTile {
+    uint32_t w[kRawWords];
+};
+
+template <int KIND>
+VSHIP_INLINE void raw_tile_kind(const FrontJob& job, int k, const FrontColor& fc, int tile_x, int tile_y,
+                                RawTile& rt)
+{
+    uint32_t* r = rt.w;
+    const uint8_t* const* plane = job.plane[k];
+    const int64_t* pstride = job.stride[k];
+    const int t = threadIdx.x;
+    const int tx = t % kTX, ty = t / kTX;
+    const int W = job.width, H = job.height;
+    const int x0 = tile_x * kFrontTileW + kBW * tx;
+    const int y0 = tile_y * kFrontTileH + kBH * ty;
+    const bool sub = fc.subw || fc.subh;
+    // the thread's 2x2 block of plane p into r[4p .. 4p + 3] (p is a constant
+    // at every call, so the words stay in registers)
+    auto block = [&](const int p) {
+        const bool aligned = plane_aligned<KIND>(plane[p], pstride[p]);
+#pragma unroll
+        for (int j = 0; j < kBH; j++) {
+            const uint8_t* row = plane[p] + (int64_t)min(y0 + j, H - 1) * pstride[p];
+#pragma unroll
+            for (int i = 0; i < kBW; i++)
+                r[4 * p + j * kBW + i] = raw_kind<KIND>(row + min(x0 + i, W - 1) * kind_bytes<KIND>(), aligned);
+        }
+    };
+    block(0);
+    if (!sub) {
+        block(1);
+        block(2);
+    }
+    if (sub) {
+        const int ccols = (kFrontTileW >> fc.subw) + 4, crows = (kFrontTileH >> fc.subh) + 4;
+        const int cx0 = ((tile_x * kFrontTileW) >> fc.subw) - 2, cy0 = ((tile_y * kFrontTileH) >> fc.subh) - 2;
+        const int cw = ((W - 1) >> fc.subw) + 1, ch = ((H - 1) >> fc.subh) + 1;
+        const int plane_elems = ccols * crows;
+        const float inv_cols = 1.0f / (float)ccols;
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            const bool aligned = plane_aligned<KIND>(plane[1 + p], pstride[1 + p]);
+#pragma unroll
+            for (int it = 0; it < kChromaIters; it++) {
+                const int e = t + it * kFrontThreads;
+                uint32_t v = 0;
+                if (e < plane_elems) {
+                    const int rr = (int)(((float)e + 0.5f) * inv_cols);
+                    const int c = e - rr * ccols;
+                    const uint8_t* row = plane[1 + p] + (int64_t)clampi(cy0 + rr, 0, ch - 1) * pstride[1 + p];
+                    v = raw_kind<KIND>(row + clampi(cx0 + c, 0, cw - 1) * kind_bytes<KIND>(), aligned);
+                }
+                r[4 + 3 * p + it] = v;
+            }
+        }
+        // both branches store words 4..11 in the same order, so that merged
+        // stores keep constant word indices and the words stay in registers
+        r[10] = 0;
+        r[11] = 0;
+    }
+}
+
+VSHIP_INLINE RawTile raw_tile(const FrontJob& job, int k, const FrontColor& fc, int tile_x, int tile_y)
+{
+    RawTile r;
+    switch (fc.sample) {
+    case kSampleU8: raw_tile_kind<kSampleU8>(job, k, fc, tile_x, tile_y, r); break;
+    case kSampleU16: raw_tile_kind<kSampleU16>(job, k, fc, tile_x, tile_y, r); break;
+    case kSampleHalf: raw_tile_kind<kSampleHalf>(job, k, fc, tile_x, tile_y, r); break;
+    default: raw_tile_kind<kSampleFloat>(job, k, fc, tile_x, tile_y, r); break;
+    }
+    return r;
+}
+
+// Plane values of the thread's block (range applied) and the staged chroma
+// samples in LDS, from raw samples.
+template <int KIND>
+VSHIP_INLINE void unpack_raw_kind(const FrontColor& fc, const RawTile& rt, float* s_chroma, float P[3][kBH][kBW])
+{
+    const uint32_t* r = rt.w;
+    const RangeXform r0 = {fc.range0[0], fc.range0[1]};
+    const RangeXform r12 = {fc.range12[0], fc.range12[1]};
+    const bool sub = fc.subw || fc.subh;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int j = 0; j < kBH; j++)
+#pragma unroll
+            for (int i = 0; i < kBW; i++)
+                if (p == 0 || !sub)
+                    P[p][j][i] = apply_range(convert_kind<KIND>(r[4 * p + j * kBW + i], fc.mask), p == 0 ? r0 : r12);
+    if (sub) {
+        const int t = threadIdx.x;
+        const int plane_elems = ((kFrontTileW >> fc.subw) + 4) * ((kFrontTileH >> fc.subh) + 4);
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int it = 0; it < kChromaIters; it++) {
+                const int e = t + it * kFrontThreads;
+                if (e < plane_elems)
+                    s_chroma[p * plane_elems + e] = apply_range(convert_kind<KIND>(r[4 + 3 * p + it], fc.mask), r12);
+            }
+    }
+}
+
+VSHIP_INLINE void unpack_raw(const FrontColor& fc, const RawTile& r, float* s_chroma, float P[3][kBH][kBW])
+{
+    switch (fc.sample) {
+    case kSampleU8: unpack_raw_kind<kSampleU8>(fc, r, s_chroma, P); break;
+    case kSampleU16: unpack_raw_kind<kSampleU16>(fc, r, s_chroma, P); break;
+    case kSampleHalf: unpack_raw_kind<kSampleHalf>(fc, r, s_chroma, P); break;
+    default: unpack_raw_kind<kSampleFloat>(fc, r, s_chroma, P); break;
+    }
+}
+
 // EOTF of the thread's block, one code path per transfer.
 template <int TRC>
 VSHIP_INLINE void eotf_block(float P[3][kBH][kBW])
@@ -228,8 +360,7 @@ struct TileXyb {
     float y0[kBH][kBW];              // level 0: Y of the thread's block
     float x1[kBH / 2], y1[kBH / 2];  // level 1: X, Y of the block's parents
     float x2 = 0.0f, y2 = 0.0f;      // level 2 (threads < 128)
-    float l3[3] = {};                // level 3 (threads < 32): X, Y, B
-    float l4[3] = {};                // level 4 (threads < 8): X, Y, B
+    float l3[3] = {};                // levels 3 and 4 (lanes 0..19 of waves 0, 1): X, Y, B
 };
 
 // Flat field index of a TileXyb value (the FP16 path's LDS stash is
@@ -240,8 +371,7 @@ constexpr int kFieldY1 = kFieldX1 + kBH / 2;
 constexpr int kFieldX2 = kFieldY1 + kBH / 2;
 constexpr int kFieldY2 = kFieldX2 + 1;
 constexpr int kFieldL3 = kFieldY2 + 1;
-constexpr int kFieldL4 = kFieldL3 + 3;
-constexpr int kFields = kFieldL4 + 3;
+constexpr int kFields = kFieldL3 + 3;
 static_assert(kFields * kFrontThreads == kFrontStashFloats, "front-end stash size");
 
 VSHIP_INLINE float field(const TileXyb& r, int f)
@@ -251,8 +381,7 @@ VSHIP_INLINE float field(const TileXyb& r, int f)
     if (f < kFieldX2) return r.y1[f - kFieldY1];
     if (f == kFieldX2) return r.x2;
     if (f == kFieldY2) return r.y2;
-    if (f < kFieldL4) return r.l3[f - kFieldL3];
-    return r.l4[f - kFieldL4];
+    return r.l3[f - kFieldL3];
 }
 
 // Level s+1 pixel from the 2x2 parents at (2lx, 2ly) of a level stored in LDS
@@ -370,8 +499,11 @@ VSHIP_INLINE void chroma_2x(const FrontColor& fc, const float* s_chroma, float* 
 // Linear BT.709 RGB of the thread's 2x2 block of image k: sample decode,
 // range, chroma upsampling (LDS), matrix, EOTF, linear-light matrix and
 // primaries. Ends with the LDS free for reuse only after the caller syncs.
+// preloaded: P's plane values and the staged chroma samples are already in
+// place (unpack_raw of samples loaded ahead by raw_tile); otherwise they are
+// loaded here.
 VSHIP_INLINE void to_linear(const FrontJob& job, int k, const FrontColor& fc, int tile_x, int tile_y, float* lds,
-                            float P[3][kBH][kBW])
+                            float P[3][kBH][kBW], bool preloaded = false)
 {
     const uint8_t* const* plane = job.plane[k];
     const int64_t* pstride = job.stride[k];
@@ -385,56 +517,66 @@ VSHIP_INLINE void to_linear(const FrontJob& job, int k, const FrontColor& fc, in
 
     // P: plane values, then linear R, G, B of the block
     const bool sub = fc.subw || fc.subh;
-    switch (fc.sample) {
-    case kSampleU8:
-        load_block<kSampleU8>(plane[0], pstride[0], x0, y0, W, H, fc.mask, r0, P[0]);
-        if (!sub) {
-            load_block<kSampleU8>(plane[1], pstride[1], x0, y0, W, H, fc.mask, r12, P[1]);
-            load_block<kSampleU8>(plane[2], pstride[2], x0, y0, W, H, fc.mask, r12, P[2]);
+    if (!preloaded) {
+        switch (fc.sample) {
+        case kSampleU8:
+            load_block<kSampleU8>(plane[0], pstride[0], x0, y0, W, H, fc.mask, r0, P[0]);
+            if (!sub) {
+                load_block<kSampleU8>(plane[1], pstride[1], x0, y0, W, H, fc.mask, r12, P[1]);
+                load_block<kSampleU8>(plane[2], pstride[2], x0, y0, W, H, fc.mask, r12, P[2]);
+            }
+            break;
+        case kSampleU16:
+            load_block<kSampleU16>(plane[0], pstride[0], x0, y0, W, H, fc.mask, r0, P[0]);
+            if (!sub) {
+                load_block<kSampleU16>(plane[1], pstride[1], x0, y0, W, H, fc.mask, r12, P[1]);
+                load_block<kSampleU16>(plane[2], pstride[2], x0, y0, W, H, fc.mask, r12, P[2]);
+            }
+            break;
+        case kSampleHalf:
+            load_block<kSampleHalf>(plane[0], pstride[0], x0, y0, W, H, fc.mask, r0, P[0]);
+            if (!sub) {
+                load_block<kSampleHalf>(plane[1], pstride[1], x0, y0, W, H, fc.mask, r12, P[1]);
+                load_block<kSampleHalf>(plane[2], pstride[2], x0, y0, W, H, fc.mask, r12, P[2]);
+            }
+            break;
+        default:
+            load_block<kSampleFloat>(plane[0], pstride[0], x0, y0, W, H, fc.mask, r0, P[0]);
+            if (!sub) {
+                load_block<kSampleFloat>(plane[1], pstride[1], x0, y0, W, H, fc.mask, r12, P[1]);
+                load_block<kSampleFloat>(plane[2], pstride[2], x0, y0, W, H, fc.mask, r12, P[2]);
+            }
+            break;
         }
-        break;
-    case kSampleU16:
-        load_block<kSampleU16>(plane[0], pstride[0], x0, y0, W, H, fc.mask, r0, P[0]);
-        if (!sub) {
-            load_block<kSampleU16>(plane[1], pstride[1], x0, y0, W, H, fc.mask, r12, P[1]);
-            load_block<kSampleU16>(plane[2], pstride[2], x0, y0, W, H, fc.mask, r12, P[2]);
+
+        if (sub) {
+            // both chroma planes staged in LDS with a 2-sample halo; clamped
+            // coordinates reproduce the reference's edge rules
+            // (chromaUpsample.hpp:29-35,41-47)
+            const int ccols = (kFrontTileW >> fc.subw) + 4;
+            const int crows = (kFrontTileH >> fc.subh) + 4;
+            const int cx0 = ((tile_x * kFrontTileW) >> fc.subw) - 2;
+            const int cy0 = ((tile_y * kFrontTileH) >> fc.subh) - 2;
+            const int cw = ((W - 1) >> fc.subw) + 1;
+            const int ch = ((H - 1) >> fc.subh) + 1;
+            switch (fc.sample) {
+            case kSampleU8: stage_chroma<kSampleU8>(plane, pstride, fc.mask, r12, lds, ccols, crows, cx0, cy0, cw, ch); break;
+            case kSampleU16: stage_chroma<kSampleU16>(plane, pstride, fc.mask, r12, lds, ccols, crows, cx0, cy0, cw, ch); break;
+            case kSampleHalf: stage_chroma<kSampleHalf>(plane, pstride, fc.mask, r12, lds, ccols, crows, cx0, cy0, cw, ch); break;
+            default: stage_chroma<kSampleFloat>(plane, pstride, fc.mask, r12, lds, ccols, crows, cx0, cy0, cw, ch); break;
+            }
         }
-        break;
-    case kSampleHalf:
-        load_block<kSampleHalf>(plane[0], pstride[0], x0, y0, W, H, fc.mask, r0, P[0]);
-        if (!sub) {
-            load_block<kSampleHalf>(plane[1], pstride[1], x0, y0, W, H, fc.mask, r12, P[1]);
-            load_block<kSampleHalf>(plane[2], pstride[2], x0, y0, W, H, fc.mask, r12, P[2]);
-        }
-        break;
-    default:
-        load_block<kSampleFloat>(plane[0], pstride[0], x0, y0, W, H, fc.mask, r0, P[0]);
-        if (!sub) {
-            load_block<kSampleFloat>(plane[1], pstride[1], x0, y0, W, H, fc.mask, r12, P[1]);
-            load_block<kSampleFloat>(plane[2], pstride[2], x0, y0, W, H, fc.mask, r12, P[2]);
-        }
-        break;
     }
 
     if (sub) {
-        // both chroma planes staged in LDS with a 2-sample halo; clamped
-        // coordinates reproduce the reference's edge rules
-        // (chromaUpsample.hpp:29-35,41-47)
+        // chroma upsampling from the staged planes
         const int ccols = (kFrontTileW >> fc.subw) + 4;
         const int crows = (kFrontTileH >> fc.subh) + 4;
         const int cx0 = ((tile_x * kFrontTileW) >> fc.subw) - 2;
         const int cy0 = ((tile_y * kFrontTileH) >> fc.subh) - 2;
-        const int cw = ((W - 1) >> fc.subw) + 1;
-        const int ch = ((H - 1) >> fc.subh) + 1;
         const int plane_elems = ccols * crows;
         float* s_chroma = lds;
         float* s_mid = lds + 2 * plane_elems;
-        switch (fc.sample) {
-        case kSampleU8: stage_chroma<kSampleU8>(plane, pstride, fc.mask, r12, s_chroma, ccols, crows, cx0, cy0, cw, ch); break;
-        case kSampleU16: stage_chroma<kSampleU16>(plane, pstride, fc.mask, r12, s_chroma, ccols, crows, cx0, cy0, cw, ch); break;
-        case kSampleHalf: stage_chroma<kSampleHalf>(plane, pstride, fc.mask, r12, s_chroma, ccols, crows, cx0, cy0, cw, ch); break;
-        default: stage_chroma<kSampleFloat>(plane, pstride, fc.mask, r12, s_chroma, ccols, crows, cx0, cy0, cw, ch); break;
-        }
         __syncthreads();
         const bool both = fc.subw && fc.subh;
         if (fc.subw <= 1 && fc.subh <= 1) {
@@ -555,16 +697,17 @@ VSHIP_INLINE void down_pairs(float v[3], bool x_out, bool y_out, FX fx, FY fy)
     }
 }
 
+// P holds the image's unpacked plane values when preloaded (see to_linear).
 VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc, const PairGeom& geo, int tile_x,
-                               int tile_y, float* lds, float* lds_l1, TileXyb& res)
+                               int tile_y, float* lds, float* lds_l1, TileXyb& res, float P[3][kBH][kBW],
+                               bool preloaded = false)
 {
     const int t = threadIdx.x;
     const int tx = t % kTX, ty = t / kTX;
     const int W = job.width, H = job.height;
     const int x0 = tile_x * kFrontTileW + kBW * tx;
     const int y0 = tile_y * kFrontTileH + kBH * ty;
-    float P[3][kBH][kBW];
-    to_linear(job, k, fc, tile_x, tile_y, lds, P);
+    to_linear(job, k, fc, tile_x, tile_y, lds, P, preloaded);
 
     // ---- level 0: Y only
 #pragma unroll
@@ -635,11 +778,8 @@ VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc,
             G[c] = __shfl(l4src ? l4m : L3[c], src, 64);
         }
         const XYB q = to_xyb(G[0], G[1], G[2], true);
-        if (lane < 16) {
-            res.l3[0] = q.x; res.l3[1] = q.y; res.l3[2] = q.b;
-        } else {
-            res.l4[0] = q.x; res.l4[1] = q.y; res.l4[2] = q.b;
-        }
+        // lanes 0..15 store them as level 3, lanes 16..19 as level 4
+        res.l3[0] = q.x; res.l3[1] = q.y; res.l3[2] = q.b;
     }
 }
 
@@ -703,7 +843,7 @@ VSHIP_INLINE void store_tile(const FrontJob& job, const PairGeom& geo, int tile_
             if (x4 < geo.w[4] && y4 < geo.h[4]) {
                 const uint32_t o = (uint32_t)y4 * (uint32_t)geo.pitch[4] + (uint32_t)x4;
 #pragma unroll
-                for (int c = 0; c < 3; c++) emit(c, (uint32_t)geo.off[4][c] + o, kFieldL4 + c);
+                for (int c = 0; c < 3; c++) emit(c, (uint32_t)geo.off[4][c] + o, kFieldL3 + c);
             }
         }
     }
@@ -759,7 +899,8 @@ VSHIP_INLINE void frontend_tile(const FrontJob& job, const FrontColor* __restric
 #pragma unroll 1
         for (int k = 0; k < 2; k++) {
             TileXyb r;
-            compute_tile(job, k, colors[job.color[k]], geo, tile_x, tile_y, work, l1_area + k * kL1Floats, r);
+            float P[3][kBH][kBW];
+            compute_tile(job, k, colors[job.color[k]], geo, tile_x, tile_y, work, l1_area + k * kL1Floats, r, P);
             void* out = job.out[k];
             store_tile(
                 job, geo, tile_x, tile_y, [&](int, uint32_t off, int f) { gstore1<StoreT>(out, off, field(r, f)); },
@@ -767,10 +908,11 @@ VSHIP_INLINE void frontend_tile(const FrontJob& job, const FrontColor* __restric
         }
     } else {
         TileXyb a, b;
-        compute_tile(job, 0, colors[job.color[0]], geo, tile_x, tile_y, work, l1_area, a);
+        float P[3][kBH][kBW];
+        compute_tile(job, 0, colors[job.color[0]], geo, tile_x, tile_y, work, l1_area, a, P);
 #pragma unroll
         for (int f = 0; f < kFields; f++) stash[f * kFrontThreads + t] = field(a, f);
-        compute_tile(job, 1, colors[job.color[1]], geo, tile_x, tile_y, work, l1_area + kL1Floats, b);
+        compute_tile(job, 1, colors[job.color[1]], geo, tile_x, tile_y, work, l1_area + kL1Floats, b, P);
         void* o0 = job.out[0];
         void* o1 = job.out[1];
         store_tile(
@@ -792,6 +934,23 @@ VSHIP_INLINE void frontend_tile(const FrontJob& job, const FrontColor* __restric
 // round-robin over the XCDs: each XCD takes a contiguous range of tiles, so
 // horizontally adjacent tiles, which share the 128-byte lines of their input
 // rows, are fetched through the same L2.
+//
+// FP16 layout: the raw samples of each image are loaded while the previous
+// image is computed (image 1 of a tile during image 0, image 0 of the next
+// tile during image 1), so the memory latency of a tile's inputs overlaps
+// work. The FP32 instantiation has no registers to spare for the 12 words in
+// flight and loads each image when it starts.
+VSHIP_INLINE int front_job_of(const FrontJob* __restrict__ jobs, int njobs, int item)
+{
+    // largest j with tile_start <= item
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile_start <= item) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 template <typename StoreT>
 __global__ __launch_bounds__(kFrontThreads, 4)
 void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ counts,
@@ -803,17 +962,64 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
     const int per = (total + kXcds - 1) / kXcds;
     const int xcd = blockIdx.x % kXcds;
     const int end = min((xcd + 1) * per, total);
-    for (int item = xcd * per + blockIdx.x / kXcds; item < end; item += gridDim.x / kXcds) {
-        // job lookup: largest j with tile_start <= item
-        int lo = 0, hi = njobs - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (jobs[mid].tile_start <= item) lo = mid; else hi = mid - 1;
+    const int step = gridDim.x / kXcds;
+    int item = xcd * per + blockIdx.x / kXcds;
+    if constexpr (sizeof(StoreT) == 4) {
+        for (; item < end; item += step) {
+            const FrontJob& job = jobs[front_job_of(jobs, njobs, item)];
+            const int local = item - job.tile_start;
+            frontend_tile<StoreT>(job, colors, geoms[job.pair], local % job.tiles_x, local / job.tiles_x, lds);
+            __syncthreads();  // LDS is reused by the next item
         }
-        const FrontJob& job = jobs[lo];
-        const int local = item - job.tile_start;
-        frontend_tile<StoreT>(job, colors, geoms[job.pair], local % job.tiles_x, local / job.tiles_x, lds);
-        __syncthreads();  // LDS is reused by the next item
+    } else {
+        float* stash = lds;
+        float* work = lds + kFrontStashFloats;
+        float* l1_area = work + kChromaLdsMax;
+        const int t = threadIdx.x;
+        RawTile raw = {};
+        if (item < end) {
+            const FrontJob& job = jobs[front_job_of(jobs, njobs, item)];
+            const int local = item - job.tile_start;
+            raw = raw_tile(job, 0, colors[job.color[0]], local % job.tiles_x, local / job.tiles_x);
+        }
+        for (; item < end; item += step) {
+            const FrontJob& job = jobs[front_job_of(jobs, njobs, item)];
+            const PairGeom& geo = geoms[job.pair];
+            const int local = item - job.tile_start;
+            const int tile_x = local % job.tiles_x, tile_y = local / job.tiles_x;
+            const FrontColor& c0 = colors[job.color[0]];
+            const FrontColor& c1 = colors[job.color[1]];
+            TileXyb a, b;
+            float P[3][kBH][kBW];
+            unpack_raw(c0, raw, work, P);
+            raw = raw_tile(job, 1, c1, tile_x, tile_y);  // in flight during image 0
+            compute_tile(job, 0, c0, geo, tile_x, tile_y, work, l1_area, a, P, true);
+#pragma unroll
+            for (int f = 0; f < kFields; f++) stash[f * kFrontThreads + t] = field(a, f);
+            unpack_raw(c1, raw, work, P);
+            const int next = item + step;
+            if (next < end) {  // the next tile's image 0, in flight during image 1
+                const FrontJob& nj = jobs[front_job_of(jobs, njobs, next)];
+                const int nl = next - nj.tile_start;
+                raw = raw_tile(nj, 0, colors[nj.color[0]], nl % nj.tiles_x, nl / nj.tiles_x);
+            }
+            compute_tile(job, 1, c1, geo, tile_x, tile_y, work, l1_area + kL1Floats, b, P, true);
+            void* o0 = job.out[0];
+            void* o1 = job.out[1];
+            store_tile(
+                job, geo, tile_x, tile_y,
+                [&](int plane, uint32_t off, int f) {
+                    const float va = stash[f * kFrontThreads + t];
+                    gstore1<StoreT>(o0, off, va - kHalfOffset[plane]);
+                    gstore1<StoreT>(o1, off, field(b, f) - va);
+                },
+                [&](int plane, uint32_t off, int f) {
+                    const float va0 = stash[f * kFrontThreads + t], va1 = stash[(f + 1) * kFrontThreads + t];
+                    gstore2<StoreT>(o0, off, va0 - kHalfOffset[plane], va1 - kHalfOffset[plane]);
+                    gstore2<StoreT>(o1, off, field(b, f) - va0, field(b, f + 1) - va1);
+                });
+            __syncthreads();  // LDS is reused by the next item
+        }
     }
 }
 

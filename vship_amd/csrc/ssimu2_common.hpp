@@ -32,7 +32,7 @@ constexpr int kDefaultBatchBlurWaves = 4;
 constexpr int kFrontTileW = 64;
 constexpr int kFrontTileH = 32;
 constexpr int kFrontThreads = 512;
-constexpr int kFrontStashFloats = 14 * kFrontThreads;  // FP16 path: reference image values per tile
+constexpr int kFrontStashFloats = 11 * kFrontThreads;  // FP16 path: reference image values per tile
 
 // Blur/map tiles: 256 threads, output tile shape chosen per launch (blur.hip),
 // halo 8 on every side.
