@@ -229,7 +229,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--handlers", type=int, default=4)
+    ap.add_argument("--handlers", type=int, default=8)  # FFVship's default -g (GPU streams)
     ap.add_argument("--cpu-pairs", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
