@@ -174,7 +174,10 @@ VSHIP_INLINE void stage_chroma(const uint8_t* const* plane, const int64_t* pstri
     // row = e / ccols from a float reciprocal: (e + 0.5) / ccols lies at least
     // 0.5 / ccols (>= 0.007) from an integer, far beyond the float error
     const float inv_cols = 1.0f / (float)ccols;
-    float v[2][kChromaIters];
+    // every load is unconditional (items past the plane reload its last
+    // sample) and the raw words are converted only after all loads are
+    // issued, so no load waits for an earlier one
+    uint32_t v[2][kChromaIters];
 #pragma unroll
     for (int p = 0; p < 2; p++) {
         const uint8_t* base = plane[1 + p];
@@ -182,14 +185,11 @@ VSHIP_INLINE void stage_chroma(const uint8_t* const* plane, const int64_t* pstri
         const bool aligned = plane_aligned<KIND>(base, stride);
 #pragma unroll
         for (int it = 0; it < kChromaIters; it++) {
-            const int e = t + it * kFrontThreads;
-            v[p][it] = 0.0f;
-            if (e < plane_elems) {
-                const int r = (int)(((float)e + 0.5f) * inv_cols);
-                const int c = e - r * ccols;
-                const uint8_t* row = base + (int64_t)clampi(cy0 + r, 0, ch - 1) * stride;
-                v[p][it] = load_kind<KIND>(row + clampi(cx0 + c, 0, cw - 1) * kind_bytes<KIND>(), mask, aligned);
-            }
+            const int e = min(t + it * kFrontThreads, plane_elems - 1);
+            const int r = (int)(((float)e + 0.5f) * inv_cols);
+            const int c = e - r * ccols;
+            const uint8_t* row = base + (int64_t)clampi(cy0 + r, 0, ch - 1) * stride;
+            v[p][it] = raw_kind<KIND>(row + clampi(cx0 + c, 0, cw - 1) * kind_bytes<KIND>(), aligned);
         }
     }
 #pragma unroll
@@ -197,7 +197,7 @@ VSHIP_INLINE void stage_chroma(const uint8_t* const* plane, const int64_t* pstri
 #pragma unroll
         for (int it = 0; it < kChromaIters; it++) {
             const int e = t + it * kFrontThreads;
-            if (e < plane_elems) s_chroma[p * plane_elems + e] = apply_range(v[p][it], rx);
+            if (e < plane_elems) s_chroma[p * plane_elems + e] = apply_range(convert_kind<KIND>(v[p][it], mask), rx);
         }
 }
 
@@ -252,15 +252,12 @@ VSHIP_INLINE void raw_tile_kind(const FrontJob& job, int k, const FrontColor& fc
             const bool aligned = plane_aligned<KIND>(plane[1 + p], pstride[1 + p]);
 #pragma unroll
             for (int it = 0; it < kChromaIters; it++) {
-                const int e = t + it * kFrontThreads;
-                uint32_t v = 0;
-                if (e < plane_elems) {
-                    const int rr = (int)(((float)e + 0.5f) * inv_cols);
-                    const int c = e - rr * ccols;
-                    const uint8_t* row = plane[1 + p] + (int64_t)clampi(cy0 + rr, 0, ch - 1) * pstride[1 + p];
-                    v = raw_kind<KIND>(row + clampi(cx0 + c, 0, cw - 1) * kind_bytes<KIND>(), aligned);
-                }
-                r[4 + 3 * p + it] = v;
+                // unconditional (items past the plane reload its last sample)
+                const int e = min(t + it * kFrontThreads, plane_elems - 1);
+                const int rr = (int)(((float)e + 0.5f) * inv_cols);
+                const int c = e - rr * ccols;
+                const uint8_t* row = plane[1 + p] + (int64_t)clampi(cy0 + rr, 0, ch - 1) * pstride[1 + p];
+                r[4 + 3 * p + it] = raw_kind<KIND>(row + clampi(cx0 + c, 0, cw - 1) * kind_bytes<KIND>(), aligned);
             }
         }
         // both branches store words 4..11 in the same order, so that merged
