@@ -376,6 +376,8 @@ Ssimu2Handler::Ssimu2Handler(const Vship_Colorspace_t& src, const Vship_Colorspa
     const FrontColor lin_fc = linear_rgb_color();
     front_lds_ = std::max(std::max(frontend_lds_bytes(spec_[0].fc), frontend_lds_bytes(spec_[1].fc)),
                           frontend_lds_bytes(lin_fc));
+    // resampled images enter the pyramid as linear RGB: generic front-end
+    front_spec_ = (spec_[0].resample || spec_[1].resample) ? kFrontGeneric : front_spec(spec_[0].fc, spec_[1].fc);
 
     VSHIP_HIP_CHECK(hipSetDevice(device_));
     VSHIP_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
@@ -529,7 +531,7 @@ void Ssimu2Handler::launch_all()
     }
     // the job (plane pointers of this call) travels as a kernel argument
     VSHIP_HIP_CHECK(launch_frontend_single<float>(*h_front_jobs_, layout_.front_tiles_per_image, d_colors_, d_geom_,
-                                                  stream_));
+                                                  front_spec_, stream_));
     VSHIP_HIP_CHECK(launch_blur<float>(layout_.blur_config, d_maps_, d_counts_, blur_grid_for(layout_.blur_tiles),
                                        taps_, d_partials_, stream_));
     // the score goes straight to the pinned host word the caller reads
@@ -621,7 +623,7 @@ void Ssimu2Handler::profile(int iters, double* ms_front, double* ms_blur, double
     for (int it = 0; it < iters; it++) {
         VSHIP_HIP_CHECK(hipEventRecord(ev[0], stream_));
         VSHIP_HIP_CHECK(launch_frontend_single<float>(*h_front_jobs_, layout_.front_tiles_per_image, d_colors_,
-                                                      d_geom_, stream_));
+                                                      d_geom_, front_spec_, stream_));
         VSHIP_HIP_CHECK(hipEventRecord(ev[1], stream_));
         VSHIP_HIP_CHECK(launch_blur<float>(layout_.blur_config, d_maps_, d_counts_, blur_grid_for(layout_.blur_tiles),
                                            taps_, d_partials_, stream_));

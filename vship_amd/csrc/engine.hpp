@@ -91,6 +91,7 @@ private:
     BlurTaps taps_;
     int width_, height_;
     size_t front_lds_ = 0;
+    int front_spec_ = 0;                      // FrontSpec of the single-pair front-end
 
     void* xyb_[2] = {nullptr, nullptr};       // float planes per image
     FrontColor* d_colors_ = nullptr;

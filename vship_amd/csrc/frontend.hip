@@ -878,12 +878,30 @@ VSHIP_INLINE void gstore2<__half>(void* base, uint32_t off, float v0, float v1)
         f16x2_t{(_Float16)v0, (_Float16)v1};
 }
 
+// Colour layouts with their own instantiation of the single-pair front-end:
+// with the sample type, subsampling, transfer and matrix steps known at
+// compile time the per-format switches disappear and the kernel needs fewer
+// registers. The descriptor's values (ranges, matrices, weights) stay
+// run-time.
+template <int SPEC>
+VSHIP_INLINE FrontColor front_color(const FrontColor& in)
+{
+    FrontColor fc = in;
+    fc.sample = SPEC == kFrontU16Yuv420Bt709 ? kSampleU16 : kSampleU8;
+    fc.subw = 1;
+    fc.subh = 1;
+    fc.transfer = 1;
+    fc.use_pre = 1;
+    fc.use_post = 0;
+    return fc;
+}
+
 // One tile of both images of a pair. FP32 layout: each image is written as
 // soon as it is computed. FP16 layout: the reference plane holds
 // fp16(a - offset) and the distorted plane fp16(b - a) of the FP32 values, so
 // the reference image's values wait in an LDS stash (field-major, one float
 // per thread and field) while the distorted image is computed.
-template <typename StoreT>
+template <typename StoreT, int SPEC = kFrontGeneric>
 VSHIP_INLINE void frontend_tile(const FrontJob& job, const FrontColor* __restrict__ colors, const PairGeom& geo,
                                 int tile_x, int tile_y, float* lds)
 {
@@ -897,7 +915,12 @@ VSHIP_INLINE void frontend_tile(const FrontJob& job, const FrontColor* __restric
         for (int k = 0; k < 2; k++) {
             TileXyb r;
             float P[3][kBH][kBW];
-            compute_tile(job, k, colors[job.color[k]], geo, tile_x, tile_y, work, l1_area + k * kL1Floats, r, P);
+            if constexpr (SPEC == kFrontGeneric) {
+                compute_tile(job, k, colors[job.color[k]], geo, tile_x, tile_y, work, l1_area + k * kL1Floats, r, P);
+            } else {
+                const FrontColor fc = front_color<SPEC>(colors[job.color[k]]);
+                compute_tile(job, k, fc, geo, tile_x, tile_y, work, l1_area + k * kL1Floats, r, P);
+            }
             void* out = job.out[k];
             store_tile(
                 job, geo, tile_x, tile_y, [&](int, uint32_t off, int f) { gstore1<StoreT>(out, off, field(r, f)); },
@@ -1023,7 +1046,7 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
 // One pair whose job travels as a kernel argument (the single-pair handler:
 // no upload of the job before each call). One workgroup per tile, in the
 // same XCD-contiguous order.
-template <typename StoreT>
+template <typename StoreT, int SPEC>
 __global__ __launch_bounds__(kFrontThreads, 4)
 void frontend_single_kernel(FrontJob job, int tiles, const FrontColor* __restrict__ colors,
                             const PairGeom* __restrict__ geoms)
@@ -1032,7 +1055,7 @@ void frontend_single_kernel(FrontJob job, int tiles, const FrontColor* __restric
     const int per = (tiles + kXcds - 1) / kXcds;
     const int item = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
     if (item >= tiles || (int)(blockIdx.x / kXcds) >= per) return;
-    frontend_tile<StoreT>(job, colors, geoms[job.pair], item % job.tiles_x, item / job.tiles_x, lds);
+    frontend_tile<StoreT, SPEC>(job, colors, geoms[job.pair], item % job.tiles_x, item / job.tiles_x, lds);
 }
 
 // Linear RGB of image 0 of one job, stored as three W x H planes with row
@@ -1113,20 +1136,44 @@ hipError_t launch_frontend(const FrontJob* jobs, const int* counts, int grid, co
     return hipGetLastError();
 }
 
+int front_spec(const FrontColor& a, const FrontColor& b)
+{
+    auto fixed = [](const FrontColor& c) {
+        return c.subw == 1 && c.subh == 1 && c.transfer == 1 && c.use_pre == 1 && c.use_post == 0;
+    };
+    if (!fixed(a) || !fixed(b) || a.sample != b.sample) return kFrontGeneric;
+    if (a.sample == kSampleU8) return kFrontU8Yuv420Bt709;
+    if (a.sample == kSampleU16) return kFrontU16Yuv420Bt709;
+    return kFrontGeneric;
+}
+
 template <typename StoreT>
 hipError_t launch_frontend_single(const FrontJob& job, int tiles, const FrontColor* colors, const PairGeom* geoms,
-                                  hipStream_t stream)
+                                  int spec, hipStream_t stream)
 {
     if (tiles <= 0) return hipSuccess;
     const int grid = (tiles + kXcds - 1) / kXcds * kXcds;
     const size_t bytes = sizeof(float) * ((size_t)kChromaLdsMax + 2 * kL1Floats +
                                           (sizeof(StoreT) == 2 ? (size_t)kFrontStashFloats : 0));
-    hipLaunchKernelGGL(frontend_single_kernel<StoreT>, dim3(grid), dim3(kFrontThreads), bytes, stream, job, tiles,
-                       colors, geoms);
+    switch (spec) {
+    case kFrontU8Yuv420Bt709:
+        hipLaunchKernelGGL((frontend_single_kernel<StoreT, kFrontU8Yuv420Bt709>), dim3(grid), dim3(kFrontThreads),
+                           bytes, stream, job, tiles, colors, geoms);
+        break;
+    case kFrontU16Yuv420Bt709:
+        hipLaunchKernelGGL((frontend_single_kernel<StoreT, kFrontU16Yuv420Bt709>), dim3(grid), dim3(kFrontThreads),
+                           bytes, stream, job, tiles, colors, geoms);
+        break;
+    default:
+        hipLaunchKernelGGL((frontend_single_kernel<StoreT, kFrontGeneric>), dim3(grid), dim3(kFrontThreads), bytes,
+                           stream, job, tiles, colors, geoms);
+        break;
+    }
     return hipGetLastError();
 }
 
-template hipError_t launch_frontend_single<float>(const FrontJob&, int, const FrontColor*, const PairGeom*, hipStream_t);
+template hipError_t launch_frontend_single<float>(const FrontJob&, int, const FrontColor*, const PairGeom*, int,
+                                                  hipStream_t);
 
 template hipError_t launch_frontend<float>(const FrontJob*, const int*, int, const FrontColor*, const PairGeom*, size_t, hipStream_t);
 template hipError_t launch_frontend<__half>(const FrontJob*, const int*, int, const FrontColor*, const PairGeom*, size_t, hipStream_t);

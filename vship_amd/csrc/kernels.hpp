@@ -16,11 +16,22 @@ template <typename StoreT>
 hipError_t launch_frontend(const FrontJob* jobs, const int* counts, int grid, const FrontColor* colors,
                            const PairGeom* geoms, size_t lds_bytes, hipStream_t stream);
 
+// Colour layouts with a front-end instantiation of their own (8- and 16-bit
+// YUV 4:2:0, BT.709 transfer, no linear-light matrix); kFrontGeneric
+// handles every layout.
+enum FrontSpec : int {
+    kFrontGeneric = 0,
+    kFrontU8Yuv420Bt709 = 1,
+    kFrontU16Yuv420Bt709 = 2,
+};
+// The instantiation serving a pair of images with these descriptors.
+int front_spec(const FrontColor& a, const FrontColor& b);
+
 // One pair's tiles (tiles = the job's 64x32 tile count) with the job passed
 // by value as a kernel argument, one workgroup per tile.
 template <typename StoreT>
 hipError_t launch_frontend_single(const FrontJob& job, int tiles, const FrontColor* colors, const PairGeom* geoms,
-                                  hipStream_t stream);
+                                  int spec, hipStream_t stream);
 
 // Linear RGB of image 0 of `job` (tiles = job's 64x32 tile count) into three
 // planes of job.width x job.height floats, row pitch `pitch` elements.
