@@ -147,6 +147,18 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
         }
         btiles += L.blur_tiles;
     }
+    // one front-end instantiation for the whole table: the layout every pair
+    // shares, else the generic one (a change re-captures the graph)
+    int spec = -1;
+    for (const FrontJob& j : fjobs_) {
+        const int sj = front_spec(colors_[j.color[0]], colors_[j.color[1]]);
+        spec = (spec == -1 || spec == sj) ? sj : kFrontGeneric;
+    }
+    if (spec < 0) spec = kFrontGeneric;
+    if (spec != front_spec_) {
+        front_spec_ = spec;
+        graph_valid_ = false;
+    }
     xyb_elems_ = elems;
     front_tiles_ = ftiles;
     blur_tiles_ = btiles;
@@ -210,10 +222,10 @@ void Ssimu2Batch::launch_kernels(hipStream_t s)
     const int bgrid = (std::max(1, tuning_.blur_waves) * cus_ + 7) / 8 * 8;
     const size_t lds = frontend_max_lds_bytes();
     if (fp16_) {
-        VSHIP_HIP_CHECK(launch_frontend<__half>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, s));
+        VSHIP_HIP_CHECK(launch_frontend<__half>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, front_spec_, s));
         VSHIP_HIP_CHECK(launch_blur<__half>(tuning_.blur_config, d_maps_, d_counts_, bgrid, taps_, partials_, s));
     } else {
-        VSHIP_HIP_CHECK(launch_frontend<float>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, s));
+        VSHIP_HIP_CHECK(launch_frontend<float>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, front_spec_, s));
         VSHIP_HIP_CHECK(launch_blur<float>(tuning_.blur_config, d_maps_, d_counts_, bgrid, taps_, partials_, s));
     }
     VSHIP_HIP_CHECK(launch_score(d_sjobs_, d_counts_, kScoreGrid, d_maps_, partials_, d_features_, d_scores_, s));
@@ -323,8 +335,12 @@ void Ssimu2Batch::profile(int iters, double* ms_front, double* ms_blur, double* 
     double acc[3] = {0, 0, 0};
     for (int it = 0; it < iters; it++) {
         VSHIP_HIP_CHECK(hipEventRecord(ev[0], stream_));
-        if (fp16_) VSHIP_HIP_CHECK(launch_frontend<__half>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, stream_));
-        else VSHIP_HIP_CHECK(launch_frontend<float>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, stream_));
+        if (fp16_)
+            VSHIP_HIP_CHECK(
+                launch_frontend<__half>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, front_spec_, stream_));
+        else
+            VSHIP_HIP_CHECK(
+                launch_frontend<float>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, front_spec_, stream_));
         VSHIP_HIP_CHECK(hipEventRecord(ev[1], stream_));
         if (fp16_) VSHIP_HIP_CHECK(launch_blur<__half>(tuning_.blur_config, d_maps_, d_counts_, bgrid, taps_, partials_, stream_));
         else VSHIP_HIP_CHECK(launch_blur<float>(tuning_.blur_config, d_maps_, d_counts_, bgrid, taps_, partials_, stream_));

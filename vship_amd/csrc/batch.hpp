@@ -106,6 +106,7 @@ private:
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
     bool graph_valid_ = false;
+    int front_spec_ = 0;  // FrontSpec of the front-end instantiation in use
 };
 
 }  // namespace vship

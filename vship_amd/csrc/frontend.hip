@@ -971,8 +971,10 @@ VSHIP_INLINE int front_job_of(const FrontJob* __restrict__ jobs, int njobs, int 
     return lo;
 }
 
-template <typename StoreT>
-__global__ __launch_bounds__(kFrontThreads, 4)
+// The layout instantiations need fewer registers: 80 give 6 waves per SIMD,
+// i.e. three 8-wave workgroups per CU (LDS allows three in both layouts).
+template <typename StoreT, int SPEC>
+__global__ __launch_bounds__(kFrontThreads, SPEC == kFrontGeneric ? 4 : 6)
 void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ counts,
                      const FrontColor* __restrict__ colors, const PairGeom* __restrict__ geoms)
 {
@@ -988,7 +990,7 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
         for (; item < end; item += step) {
             const FrontJob& job = jobs[front_job_of(jobs, njobs, item)];
             const int local = item - job.tile_start;
-            frontend_tile<StoreT>(job, colors, geoms[job.pair], local % job.tiles_x, local / job.tiles_x, lds);
+            frontend_tile<StoreT, SPEC>(job, colors, geoms[job.pair], local % job.tiles_x, local / job.tiles_x, lds);
             __syncthreads();  // LDS is reused by the next item
         }
     } else {
@@ -996,19 +998,24 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
         float* work = lds + kFrontStashFloats;
         float* l1_area = work + kChromaLdsMax;
         const int t = threadIdx.x;
+        // descriptor of image color index i, with the layout's constants folded in
+        auto color = [&](int i) {
+            if constexpr (SPEC == kFrontGeneric) return colors[i];
+            else return front_color<SPEC>(colors[i]);
+        };
         RawTile raw = {};
         if (item < end) {
             const FrontJob& job = jobs[front_job_of(jobs, njobs, item)];
             const int local = item - job.tile_start;
-            raw = raw_tile(job, 0, colors[job.color[0]], local % job.tiles_x, local / job.tiles_x);
+            raw = raw_tile(job, 0, color(job.color[0]), local % job.tiles_x, local / job.tiles_x);
         }
         for (; item < end; item += step) {
             const FrontJob& job = jobs[front_job_of(jobs, njobs, item)];
             const PairGeom& geo = geoms[job.pair];
             const int local = item - job.tile_start;
             const int tile_x = local % job.tiles_x, tile_y = local / job.tiles_x;
-            const FrontColor& c0 = colors[job.color[0]];
-            const FrontColor& c1 = colors[job.color[1]];
+            const FrontColor c0 = color(job.color[0]);
+            const FrontColor c1 = color(job.color[1]);
             TileXyb a, b;
             float P[3][kBH][kBW];
             unpack_raw(c0, raw, work, P);
@@ -1021,7 +1028,7 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
             if (next < end) {  // the next tile's image 0, in flight during image 1
                 const FrontJob& nj = jobs[front_job_of(jobs, njobs, next)];
                 const int nl = next - nj.tile_start;
-                raw = raw_tile(nj, 0, colors[nj.color[0]], nl % nj.tiles_x, nl / nj.tiles_x);
+                raw = raw_tile(nj, 0, color(nj.color[0]), nl % nj.tiles_x, nl / nj.tiles_x);
             }
             compute_tile(job, 1, c1, geo, tile_x, tile_y, work, l1_area + kL1Floats, b, P, true);
             void* o0 = job.out[0];
@@ -1047,7 +1054,7 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
 // no upload of the job before each call). One workgroup per tile, in the
 // same XCD-contiguous order.
 template <typename StoreT, int SPEC>
-__global__ __launch_bounds__(kFrontThreads, 4)
+__global__ __launch_bounds__(kFrontThreads, SPEC == kFrontGeneric ? 4 : 6)
 void frontend_single_kernel(FrontJob job, int tiles, const FrontColor* __restrict__ colors,
                             const PairGeom* __restrict__ geoms)
 {
@@ -1123,7 +1130,7 @@ size_t frontend_max_lds_bytes()
 
 template <typename StoreT>
 hipError_t launch_frontend(const FrontJob* jobs, const int* counts, int grid, const FrontColor* colors,
-                           const PairGeom* geoms, size_t lds_bytes, hipStream_t stream)
+                           const PairGeom* geoms, size_t lds_bytes, int spec, hipStream_t stream)
 {
     if (grid <= 0) return hipSuccess;
     grid = (grid + kXcds - 1) / kXcds * kXcds;  // the item mapping needs whole XCD rounds
@@ -1131,8 +1138,20 @@ hipError_t launch_frontend(const FrontJob* jobs, const int* counts, int grid, co
     (void)lds_bytes;
     const size_t bytes = sizeof(float) * ((size_t)kChromaLdsMax + 2 * kL1Floats +
                                           (sizeof(StoreT) == 2 ? (size_t)kFrontStashFloats : 0));
-    hipLaunchKernelGGL(frontend_kernel<StoreT>, dim3(grid), dim3(kFrontThreads), bytes, stream,
-                       jobs, counts, colors, geoms);
+    switch (spec) {
+    case kFrontU8Yuv420Bt709:
+        hipLaunchKernelGGL((frontend_kernel<StoreT, kFrontU8Yuv420Bt709>), dim3(grid), dim3(kFrontThreads), bytes,
+                           stream, jobs, counts, colors, geoms);
+        break;
+    case kFrontU16Yuv420Bt709:
+        hipLaunchKernelGGL((frontend_kernel<StoreT, kFrontU16Yuv420Bt709>), dim3(grid), dim3(kFrontThreads), bytes,
+                           stream, jobs, counts, colors, geoms);
+        break;
+    default:
+        hipLaunchKernelGGL((frontend_kernel<StoreT, kFrontGeneric>), dim3(grid), dim3(kFrontThreads), bytes, stream,
+                           jobs, counts, colors, geoms);
+        break;
+    }
     return hipGetLastError();
 }
 
@@ -1175,7 +1194,9 @@ hipError_t launch_frontend_single(const FrontJob& job, int tiles, const FrontCol
 template hipError_t launch_frontend_single<float>(const FrontJob&, int, const FrontColor*, const PairGeom*, int,
                                                   hipStream_t);
 
-template hipError_t launch_frontend<float>(const FrontJob*, const int*, int, const FrontColor*, const PairGeom*, size_t, hipStream_t);
-template hipError_t launch_frontend<__half>(const FrontJob*, const int*, int, const FrontColor*, const PairGeom*, size_t, hipStream_t);
+template hipError_t launch_frontend<float>(const FrontJob*, const int*, int, const FrontColor*, const PairGeom*, size_t,
+                                           int, hipStream_t);
+template hipError_t launch_frontend<__half>(const FrontJob*, const int*, int, const FrontColor*, const PairGeom*, size_t,
+                                            int, hipStream_t);
 
 }  // namespace vship

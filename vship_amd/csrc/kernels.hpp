@@ -10,12 +10,6 @@ namespace vship {
 size_t frontend_lds_bytes(const FrontColor& fc);
 size_t frontend_max_lds_bytes();
 
-// counts: device array indexed by CountSlot. With fewer workgroups than items
-// the kernels loop over items (persistent mode).
-template <typename StoreT>
-hipError_t launch_frontend(const FrontJob* jobs, const int* counts, int grid, const FrontColor* colors,
-                           const PairGeom* geoms, size_t lds_bytes, hipStream_t stream);
-
 // Colour layouts with a front-end instantiation of their own (8- and 16-bit
 // YUV 4:2:0, BT.709 transfer, no linear-light matrix); kFrontGeneric
 // handles every layout.
@@ -26,6 +20,15 @@ enum FrontSpec : int {
 };
 // The instantiation serving a pair of images with these descriptors.
 int front_spec(const FrontColor& a, const FrontColor& b);
+
+// counts: device array indexed by CountSlot. With fewer workgroups than items
+// the kernels loop over items (persistent mode).
+// spec: the FrontSpec every image of the job table shares (kFrontGeneric
+// when they differ).
+template <typename StoreT>
+hipError_t launch_frontend(const FrontJob* jobs, const int* counts, int grid, const FrontColor* colors,
+                           const PairGeom* geoms, size_t lds_bytes, int spec, hipStream_t stream);
+
 
 // One pair's tiles (tiles = the job's 64x32 tile count) with the job passed
 // by value as a kernel argument, one workgroup per tile.
