@@ -104,6 +104,45 @@ def test_graph_replay_with_changing_compositions(pairs):
     assert one[0] == first[5]
 
 
+@pytest.mark.parametrize("kind", [0, 1])  # 8-bit and 10-bit YUV 4:2:0 BT.709 limited
+def test_layout_instantiation_batch_vs_oracle(kind):
+    """A batch whose images all share the 8- or 16-bit YUV 4:2:0 BT.709 layout
+    runs the front-end instantiation compiled for that layout: FP32 storage
+    equals the single-pair handler bit for bit and is within 0.005 of the FP64
+    oracle; FP16 storage equals the generic front-end bit for bit (forced by
+    adding a pair of another layout to the batch) and is within 0.01 of the oracle on
+    images of 64x64 and more; identity stays 100."""
+    sizes = [(131, 75), (64, 64), (200, 117), (17, 9), (1, 1), (320, 180), (40, 30)]
+    n = len(sizes) - 1
+    dp = DevicePairs(sizes, [kind] * n + [17], seed=40 + kind)  # last pair: 8-bit RGB, never specialised
+    try:
+        with S.SSIMU2Batch(8, S.STORAGE_FP32) as b:
+            b.set_tuning(HANDLER_BLUR_CONFIG, 4, 4)
+            got32 = b.compute(*dp.args(range(n)))
+        with S.SSIMU2Batch(8, S.STORAGE_FP16) as b:
+            got16 = b.compute(*dp.args(range(n)))
+            generic16 = b.compute(*dp.args())
+            cs, _, p1, _, st, _ = dp.args(range(n))
+            ident = b.compute(cs, cs, p1, p1, st, st)
+        for i, cs_i in enumerate(dp.cs[:n]):
+            ref, dist = dp.host[i]
+            with S.SSIMU2Handler(cs_i, cs_i) as h:
+                plain = h.compute(ref, dist, dp.strides[i], dp.strides[i])
+            want, _ = oracle.ssimu2(cs_i, ref, dp.strides[i], cs_i, dist, dp.strides[i])
+            _say(f"layout batch kind {kind} {cs_i.width}x{cs_i.height}: fp32 {got32[i]:.6f} plain {plain:.6f} "
+                 f"fp16 {got16[i]:.6f} generic fp16 {generic16[i]:.6f} oracle {want:.6f}")
+            assert got32[i] == plain
+            assert abs(got32[i] - want) <= 0.005
+            assert got16[i] == generic16[i]  # same arithmetic, constants folded
+            # FP16 storage rounds every stored XYB sample to 11 bits; images
+            # below 64x64 have too few pixels per scale to average that out
+            # (17x9 at 10 bits moves by about 0.02, generic front-end alike)
+            assert abs(got16[i] - want) <= (0.01 if cs_i.width * cs_i.height >= 64 * 64 else 0.05)
+        assert (ident == 100.0).all(), ident
+    finally:
+        dp.free()
+
+
 def test_identity_in_batch(pairs):
     cs, _, p1, _, st, _ = pairs.args()
     with S.SSIMU2Batch(16, S.STORAGE_FP16) as b:
