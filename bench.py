@@ -34,7 +34,7 @@ from vship_amd import synth  # noqa: E402
 W, H, PAIRS = 1920, 1080, 240
 HBM_PEAK_GBS = 8000.0
 VALU_FP32_PEAK_TFLOPS = 78.6  # non-packed FP32 FMA rate: packed FP32 is disabled in this build
-FRONT_KERNEL = "frontend_single_kernel<float>"  # the plain path's front-end (job as kernel argument)
+FRONT_KERNEL = "frontend_single_kernel<float, 1>"  # the plain path's front-end, 8-bit YUV 4:2:0 BT.709 instantiation
 BLUR_KERNEL = "blur_strip_kernel<float>"  # the plain path's blur/map kernel (matrix cores, 64x128 strips)
 
 

@@ -58,8 +58,9 @@ typedef enum {
 
 /* Tuning of the batched kernels: blur tile shape (FP32 VALU kernel 0: 64x32,
  * 1: 64x16, 2: 32x64, 3: 32x32, 4: 64x64; matrix-core kernels 5: 64x48
- * tiles, 6: 64x64, 7: 64x128, 8: 64x256 streamed strips) and persistent
- * workgroups per CU of the blur and front-end kernels. */
+ * tiles, 6: 64x64, 7: 64x128, 8: 64x256 streamed strips) and the grids of
+ * the blur and front-end kernels in workgroups per CU (1..64; workgroups loop
+ * over the items beyond the grid). */
 typedef struct VshipAMD_BatchTuning {
     int blur_config;
     int blur_waves;

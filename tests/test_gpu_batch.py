@@ -206,6 +206,13 @@ def test_batch_errors(pairs):
         with pytest.raises(S.VshipError) as e:
             b.compute(*args)
         assert e.value.code == 4
+        for bad in [(BLUR_CONFIGS, 4, 4), (8, 0, 4), (8, 65, 4), (8, 4, 0), (8, 4, 65)]:
+            with pytest.raises(S.VshipError):
+                b.set_tuning(*bad)
+        b.set_tuning(8, 64, 64)  # the largest grids, on a one-pair batch
+        one = b.compute(*pairs.args([2]))
+        b.set_tuning(8, 1, 1)
+        np.testing.assert_array_equal(b.compute(*pairs.args([2])), one)
 
 
 def test_mixed_resolution_batch_with_4k_10bit_vs_oracle():

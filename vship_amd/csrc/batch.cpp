@@ -270,8 +270,8 @@ void Ssimu2Batch::features(double* out)
 
 void Ssimu2Batch::set_tuning(const BatchTuning& t)
 {
-    if (t.blur_config < 0 || t.blur_config >= blur_tile_configs() || t.blur_waves < 1 || t.blur_waves > 16 ||
-        t.front_waves < 1 || t.front_waves > 16)
+    if (t.blur_config < 0 || t.blur_config >= blur_tile_configs() || t.blur_waves < 1 || t.blur_waves > 64 ||
+        t.front_waves < 1 || t.front_waves > 64)
         VSHIP_THROW(Vship_BadPointer, "tuning parameters out of range");
     tuning_ = t;
     graph_valid_ = false;
@@ -301,14 +301,14 @@ void Ssimu2Batch::autotune(const BatchInput& in, int reps, BatchTuning* chosen, 
     double best_t = 1e30;
     // blur tile shape and persistent blur grid, front-end grid fixed
     for (int cfg = 0; cfg < blur_tile_configs(); cfg++)
-        for (int bw : {1, 2, 3, 4}) {
+        for (int bw : {2, 4, 16, 32, 64}) {
             BatchTuning t = best_t < 1e30 ? best : tuning_;
             t.blur_config = cfg;
             t.blur_waves = bw;
             const double ms = time_config(t);
             if (ms < best_t) { best_t = ms; best = t; }
         }
-    for (int fw : {1, 2, 4, 6, 8}) {
+    for (int fw : {2, 4, 8, 16, 32, 64}) {
         BatchTuning t = best;
         t.front_waves = fw;
         const double ms = time_config(t);

@@ -26,8 +26,8 @@ namespace vship {
 
 struct BatchTuning {
     int blur_config = kDefaultBatchBlurConfig;  // blur tile shape (blur_tile_shape)
-    int blur_waves = kDefaultBatchBlurWaves;    // persistent blur workgroups per CU
-    int front_waves = 4;     // persistent front-end workgroups per CU
+    int blur_waves = kDefaultBatchBlurWaves;    // blur workgroups per CU in the grid (1..64)
+    int front_waves = kDefaultBatchFrontWaves;  // front-end workgroups per CU in the grid (1..64)
 };
 
 struct BatchInput {

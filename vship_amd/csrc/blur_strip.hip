@@ -503,17 +503,19 @@ void blur_strip_kernel(const MapJob* __restrict__ jobs, const int* __restrict__ 
     __shared__ __attribute__((aligned(16))) float ring[2 * kSlots * kSlotFloats];
     __shared__ float red[32];
     __shared__ ToeplitzLds toe;
-    if (threadIdx.x < 64) toeplitz_to_lds(make_toeplitz(taps, threadIdx.x), threadIdx.x, &toe);
-    __syncthreads();
-    float* sA = ring;
-    float* sB = ring + kSlots * kSlotFloats;
     const int total = counts[kCountBlurTiles];
     const int njobs = counts[kCountBlurJobs];
     const int per = (total + kXcds - 1) / kXcds;
     const int xcd = blockIdx.x % kXcds;
     const int slots = gridDim.x / kXcds;
     const int end = min((xcd + 1) * per, total);
-    for (int item = xcd * per + blockIdx.x / kXcds; item < end; item += slots) {
+    const int first = xcd * per + blockIdx.x / kXcds;
+    if (first >= end) return;  // a grid sized for larger batches: nothing to set up
+    if (threadIdx.x < 64) toeplitz_to_lds(make_toeplitz(taps, threadIdx.x), threadIdx.x, &toe);
+    __syncthreads();
+    float* sA = ring;
+    float* sB = ring + kSlots * kSlotFloats;
+    for (int item = first; item < end; item += slots) {
         int lo = 0, hi = njobs - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
