@@ -151,7 +151,8 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
     // shares, else the generic one (a change re-captures the graph)
     int spec = -1;
     for (const FrontJob& j : fjobs_) {
-        const int sj = front_spec(colors_[j.color[0]], colors_[j.color[1]]);
+        int sj = front_spec(colors_[j.color[0]], colors_[j.color[1]]);
+        if (sj != kFrontGeneric && !front_offsets_fit(j, colors_[j.color[0]], colors_[j.color[1]])) sj = kFrontGeneric;
         spec = (spec == -1 || spec == sj) ? sj : kFrontGeneric;
     }
     if (spec < 0) spec = kFrontGeneric;

@@ -510,6 +510,12 @@ void Ssimu2Handler::ensure_staging(int image, const int64_t ls[3])
     }
 }
 
+int Ssimu2Handler::call_spec() const
+{
+    if (front_spec_ == kFrontGeneric) return kFrontGeneric;
+    return front_offsets_fit(*h_front_jobs_, spec_[0].fc, spec_[1].fc) ? front_spec_ : kFrontGeneric;
+}
+
 void Ssimu2Handler::launch_all()
 {
     if (d_lin_jobs_) {
@@ -531,7 +537,7 @@ void Ssimu2Handler::launch_all()
     }
     // the job (plane pointers of this call) travels as a kernel argument
     VSHIP_HIP_CHECK(launch_frontend_single<float>(*h_front_jobs_, layout_.front_tiles_per_image, d_colors_, d_geom_,
-                                                  front_spec_, stream_));
+                                                  call_spec(), stream_));
     VSHIP_HIP_CHECK(launch_blur<float>(layout_.blur_config, d_maps_, d_counts_, blur_grid_for(layout_.blur_tiles),
                                        taps_, d_partials_, stream_));
     // the score goes straight to the pinned host word the caller reads
@@ -623,7 +629,7 @@ void Ssimu2Handler::profile(int iters, double* ms_front, double* ms_blur, double
     for (int it = 0; it < iters; it++) {
         VSHIP_HIP_CHECK(hipEventRecord(ev[0], stream_));
         VSHIP_HIP_CHECK(launch_frontend_single<float>(*h_front_jobs_, layout_.front_tiles_per_image, d_colors_,
-                                                      d_geom_, front_spec_, stream_));
+                                                      d_geom_, call_spec(), stream_));
         VSHIP_HIP_CHECK(hipEventRecord(ev[1], stream_));
         VSHIP_HIP_CHECK(launch_blur<float>(layout_.blur_config, d_maps_, d_counts_, blur_grid_for(layout_.blur_tiles),
                                            taps_, d_partials_, stream_));

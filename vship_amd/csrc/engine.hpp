@@ -92,6 +92,9 @@ private:
     int width_, height_;
     size_t front_lds_ = 0;
     int front_spec_ = 0;                      // FrontSpec of the single-pair front-end
+    // front_spec_, or kFrontGeneric when this call's line sizes do not fit
+    // the layout instantiation's 32-bit sample offsets
+    int call_spec() const;
 
     void* xyb_[2] = {nullptr, nullptr};       // float planes per image
     FrontColor* d_colors_ = nullptr;

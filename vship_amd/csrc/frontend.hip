@@ -140,19 +140,33 @@ VSHIP_INLINE bool plane_aligned(const uint8_t* base, int64_t stride)
 template <int KIND>
 constexpr int kind_bytes() { return KIND == kSampleU8 ? 1 : (KIND == kSampleFloat ? 4 : 2); }
 
+// Address of the sample `xb` bytes into row `row` of a plane. In general a
+// 64-bit product; with O32 the host has checked (front_offsets_fit) that the
+// stride is below 2^24 and that every row of the plane starts within 2^32
+// bytes of its base, so the whole offset is one full-rate 24-bit
+// multiply-add and the load takes the plane base as its scalar address (the
+// 64-bit product takes several quarter-rate instructions).
+template <bool O32>
+VSHIP_INLINE const uint8_t* sample_ptr(const uint8_t* base, int row, int64_t stride, int xb)
+{
+    if constexpr (O32) return base + (__umul24((uint32_t)row, (uint32_t)stride) + (uint32_t)xb);
+    else return base + (int64_t)row * stride + xb;
+}
+
 // kBH x kBW block of one plane at full resolution (clamped at the image edge).
-template <int KIND>
+template <int KIND, bool O32 = false>
 VSHIP_INLINE void load_block(const uint8_t* base, int64_t stride, int x0, int y0, int W, int H, uint32_t mask,
                              const RangeXform& rx, float out[kBH][kBW])
 {
     const bool aligned = plane_aligned<KIND>(base, stride);
 #pragma unroll
     for (int j = 0; j < kBH; j++) {
-        const uint8_t* row = base + (int64_t)min(y0 + j, H - 1) * stride;
 #pragma unroll
         for (int i = 0; i < kBW; i++)
-            out[j][i] =
-                apply_range(load_kind<KIND>(row + min(x0 + i, W - 1) * kind_bytes<KIND>(), mask, aligned), rx);
+            out[j][i] = apply_range(load_kind<KIND>(sample_ptr<O32>(base, min(y0 + j, H - 1), stride,
+                                                                    min(x0 + i, W - 1) * kind_bytes<KIND>()),
+                                                    mask, aligned),
+                                    rx);
     }
 }
 
@@ -164,7 +178,7 @@ constexpr int kChromaIters = 3;
 static_assert(kChromaIters * kFrontThreads >= 68 * 20 && kChromaIters * kFrontThreads >= 36 * 36,
               "chroma staging items per thread");
 
-template <int KIND>
+template <int KIND, bool O32 = false>
 VSHIP_INLINE void stage_chroma(const uint8_t* const* plane, const int64_t* pstride, uint32_t mask,
                                const RangeXform& rx, float* s_chroma, int ccols, int crows, int cx0, int cy0, int cw,
                                int ch)
@@ -188,8 +202,9 @@ VSHIP_INLINE void stage_chroma(const uint8_t* const* plane, const int64_t* pstri
             const int e = min(t + it * kFrontThreads, plane_elems - 1);
             const int r = (int)(((float)e + 0.5f) * inv_cols);
             const int c = e - r * ccols;
-            const uint8_t* row = base + (int64_t)clampi(cy0 + r, 0, ch - 1) * stride;
-            v[p][it] = raw_kind<KIND>(row + clampi(cx0 + c, 0, cw - 1) * kind_bytes<KIND>(), aligned);
+            v[p][it] = raw_kind<KIND>(
+                sample_ptr<O32>(base, clampi(cy0 + r, 0, ch - 1), stride, clampi(cx0 + c, 0, cw - 1) * kind_bytes<KIND>()),
+                aligned);
         }
     }
 #pragma unroll
@@ -211,7 +226,7 @@ struct RawTile {
     uint32_t w[kRawWords];
 };
 
-template <int KIND>
+template <int KIND, bool O32>
 VSHIP_INLINE void raw_tile_kind(const FrontJob& job, int k, const FrontColor& fc, int tile_x, int tile_y,
                                 RawTile& rt)
 {
@@ -230,10 +245,11 @@ VSHIP_INLINE void raw_tile_kind(const FrontJob& job, int k, const FrontColor& fc
         const bool aligned = plane_aligned<KIND>(plane[p], pstride[p]);
 #pragma unroll
         for (int j = 0; j < kBH; j++) {
-            const uint8_t* row = plane[p] + (int64_t)min(y0 + j, H - 1) * pstride[p];
 #pragma unroll
             for (int i = 0; i < kBW; i++)
-                r[4 * p + j * kBW + i] = raw_kind<KIND>(row + min(x0 + i, W - 1) * kind_bytes<KIND>(), aligned);
+                r[4 * p + j * kBW + i] = raw_kind<KIND>(
+                    sample_ptr<O32>(plane[p], min(y0 + j, H - 1), pstride[p], min(x0 + i, W - 1) * kind_bytes<KIND>()),
+                    aligned);
         }
     };
     block(0);
@@ -256,8 +272,10 @@ VSHIP_INLINE void raw_tile_kind(const FrontJob& job, int k, const FrontColor& fc
                 const int e = min(t + it * kFrontThreads, plane_elems - 1);
                 const int rr = (int)(((float)e + 0.5f) * inv_cols);
                 const int c = e - rr * ccols;
-                const uint8_t* row = plane[1 + p] + (int64_t)clampi(cy0 + rr, 0, ch - 1) * pstride[1 + p];
-                r[4 + 3 * p + it] = raw_kind<KIND>(row + clampi(cx0 + c, 0, cw - 1) * kind_bytes<KIND>(), aligned);
+                r[4 + 3 * p + it] = raw_kind<KIND>(sample_ptr<O32>(plane[1 + p], clampi(cy0 + rr, 0, ch - 1),
+                                                                   pstride[1 + p],
+                                                                   clampi(cx0 + c, 0, cw - 1) * kind_bytes<KIND>()),
+                                                   aligned);
             }
         }
         // both branches store words 4..11 in the same order, so that merged
@@ -267,14 +285,15 @@ VSHIP_INLINE void raw_tile_kind(const FrontJob& job, int k, const FrontColor& fc
     }
 }
 
+template <bool O32>
 VSHIP_INLINE RawTile raw_tile(const FrontJob& job, int k, const FrontColor& fc, int tile_x, int tile_y)
 {
     RawTile r;
     switch (fc.sample) {
-    case kSampleU8: raw_tile_kind<kSampleU8>(job, k, fc, tile_x, tile_y, r); break;
-    case kSampleU16: raw_tile_kind<kSampleU16>(job, k, fc, tile_x, tile_y, r); break;
-    case kSampleHalf: raw_tile_kind<kSampleHalf>(job, k, fc, tile_x, tile_y, r); break;
-    default: raw_tile_kind<kSampleFloat>(job, k, fc, tile_x, tile_y, r); break;
+    case kSampleU8: raw_tile_kind<kSampleU8, O32>(job, k, fc, tile_x, tile_y, r); break;
+    case kSampleU16: raw_tile_kind<kSampleU16, O32>(job, k, fc, tile_x, tile_y, r); break;
+    case kSampleHalf: raw_tile_kind<kSampleHalf, O32>(job, k, fc, tile_x, tile_y, r); break;
+    default: raw_tile_kind<kSampleFloat, O32>(job, k, fc, tile_x, tile_y, r); break;
     }
     return r;
 }
@@ -499,6 +518,7 @@ VSHIP_INLINE void chroma_2x(const FrontColor& fc, const float* s_chroma, float* 
 // preloaded: P's plane values and the staged chroma samples are already in
 // place (unpack_raw of samples loaded ahead by raw_tile); otherwise they are
 // loaded here.
+template <bool O32 = false>
 VSHIP_INLINE void to_linear(const FrontJob& job, int k, const FrontColor& fc, int tile_x, int tile_y, float* lds,
                             float P[3][kBH][kBW], bool preloaded = false)
 {
@@ -517,31 +537,31 @@ VSHIP_INLINE void to_linear(const FrontJob& job, int k, const FrontColor& fc, in
     if (!preloaded) {
         switch (fc.sample) {
         case kSampleU8:
-            load_block<kSampleU8>(plane[0], pstride[0], x0, y0, W, H, fc.mask, r0, P[0]);
+            load_block<kSampleU8, O32>(plane[0], pstride[0], x0, y0, W, H, fc.mask, r0, P[0]);
             if (!sub) {
-                load_block<kSampleU8>(plane[1], pstride[1], x0, y0, W, H, fc.mask, r12, P[1]);
-                load_block<kSampleU8>(plane[2], pstride[2], x0, y0, W, H, fc.mask, r12, P[2]);
+                load_block<kSampleU8, O32>(plane[1], pstride[1], x0, y0, W, H, fc.mask, r12, P[1]);
+                load_block<kSampleU8, O32>(plane[2], pstride[2], x0, y0, W, H, fc.mask, r12, P[2]);
             }
             break;
         case kSampleU16:
-            load_block<kSampleU16>(plane[0], pstride[0], x0, y0, W, H, fc.mask, r0, P[0]);
+            load_block<kSampleU16, O32>(plane[0], pstride[0], x0, y0, W, H, fc.mask, r0, P[0]);
             if (!sub) {
-                load_block<kSampleU16>(plane[1], pstride[1], x0, y0, W, H, fc.mask, r12, P[1]);
-                load_block<kSampleU16>(plane[2], pstride[2], x0, y0, W, H, fc.mask, r12, P[2]);
+                load_block<kSampleU16, O32>(plane[1], pstride[1], x0, y0, W, H, fc.mask, r12, P[1]);
+                load_block<kSampleU16, O32>(plane[2], pstride[2], x0, y0, W, H, fc.mask, r12, P[2]);
             }
             break;
         case kSampleHalf:
-            load_block<kSampleHalf>(plane[0], pstride[0], x0, y0, W, H, fc.mask, r0, P[0]);
+            load_block<kSampleHalf, O32>(plane[0], pstride[0], x0, y0, W, H, fc.mask, r0, P[0]);
             if (!sub) {
-                load_block<kSampleHalf>(plane[1], pstride[1], x0, y0, W, H, fc.mask, r12, P[1]);
-                load_block<kSampleHalf>(plane[2], pstride[2], x0, y0, W, H, fc.mask, r12, P[2]);
+                load_block<kSampleHalf, O32>(plane[1], pstride[1], x0, y0, W, H, fc.mask, r12, P[1]);
+                load_block<kSampleHalf, O32>(plane[2], pstride[2], x0, y0, W, H, fc.mask, r12, P[2]);
             }
             break;
         default:
-            load_block<kSampleFloat>(plane[0], pstride[0], x0, y0, W, H, fc.mask, r0, P[0]);
+            load_block<kSampleFloat, O32>(plane[0], pstride[0], x0, y0, W, H, fc.mask, r0, P[0]);
             if (!sub) {
-                load_block<kSampleFloat>(plane[1], pstride[1], x0, y0, W, H, fc.mask, r12, P[1]);
-                load_block<kSampleFloat>(plane[2], pstride[2], x0, y0, W, H, fc.mask, r12, P[2]);
+                load_block<kSampleFloat, O32>(plane[1], pstride[1], x0, y0, W, H, fc.mask, r12, P[1]);
+                load_block<kSampleFloat, O32>(plane[2], pstride[2], x0, y0, W, H, fc.mask, r12, P[2]);
             }
             break;
         }
@@ -557,10 +577,10 @@ VSHIP_INLINE void to_linear(const FrontJob& job, int k, const FrontColor& fc, in
             const int cw = ((W - 1) >> fc.subw) + 1;
             const int ch = ((H - 1) >> fc.subh) + 1;
             switch (fc.sample) {
-            case kSampleU8: stage_chroma<kSampleU8>(plane, pstride, fc.mask, r12, lds, ccols, crows, cx0, cy0, cw, ch); break;
-            case kSampleU16: stage_chroma<kSampleU16>(plane, pstride, fc.mask, r12, lds, ccols, crows, cx0, cy0, cw, ch); break;
-            case kSampleHalf: stage_chroma<kSampleHalf>(plane, pstride, fc.mask, r12, lds, ccols, crows, cx0, cy0, cw, ch); break;
-            default: stage_chroma<kSampleFloat>(plane, pstride, fc.mask, r12, lds, ccols, crows, cx0, cy0, cw, ch); break;
+            case kSampleU8: stage_chroma<kSampleU8, O32>(plane, pstride, fc.mask, r12, lds, ccols, crows, cx0, cy0, cw, ch); break;
+            case kSampleU16: stage_chroma<kSampleU16, O32>(plane, pstride, fc.mask, r12, lds, ccols, crows, cx0, cy0, cw, ch); break;
+            case kSampleHalf: stage_chroma<kSampleHalf, O32>(plane, pstride, fc.mask, r12, lds, ccols, crows, cx0, cy0, cw, ch); break;
+            default: stage_chroma<kSampleFloat, O32>(plane, pstride, fc.mask, r12, lds, ccols, crows, cx0, cy0, cw, ch); break;
             }
         }
     }
@@ -695,6 +715,7 @@ VSHIP_INLINE void down_pairs(float v[3], bool x_out, bool y_out, FX fx, FY fy)
 }
 
 // P holds the image's unpacked plane values when preloaded (see to_linear).
+template <bool O32 = false>
 VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc, const PairGeom& geo, int tile_x,
                                int tile_y, float* lds, float* lds_l1, TileXyb& res, float P[3][kBH][kBW],
                                bool preloaded = false)
@@ -704,7 +725,7 @@ VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc,
     const int W = job.width, H = job.height;
     const int x0 = tile_x * kFrontTileW + kBW * tx;
     const int y0 = tile_y * kFrontTileH + kBH * ty;
-    to_linear(job, k, fc, tile_x, tile_y, lds, P, preloaded);
+    to_linear<O32>(job, k, fc, tile_x, tile_y, lds, P, preloaded);
 
     // ---- level 0: Y only
 #pragma unroll
@@ -798,7 +819,7 @@ VSHIP_INLINE void store_tile(const FrontJob& job, const PairGeom& geo, int tile_
 #pragma unroll
     for (int j = 0; j < kBH; j++) {
         if (y0 + j < H) {
-            const uint32_t o = (uint32_t)geo.off[0][1] + (uint32_t)(y0 + j) * (uint32_t)geo.pitch[0] + (uint32_t)x0;
+            const uint32_t o = (uint32_t)geo.off[0][1] + __umul24((uint32_t)(y0 + j), (uint32_t)geo.pitch[0]) + (uint32_t)x0;
             if (x0 + 1 < W) {
                 emit2(1, o, kFieldY0 + j * kBW);
             } else if (x0 < W) {
@@ -811,7 +832,7 @@ VSHIP_INLINE void store_tile(const FrontJob& job, const PairGeom& geo, int tile_
     for (int j = 0; j < kBH / 2; j++) {
         const int y1 = tile_y * 16 + ty * (kBH / 2) + j;
         if (y1 < geo.h[1] && x1 < geo.w[1]) {
-            const uint32_t o = (uint32_t)y1 * (uint32_t)geo.pitch[1] + (uint32_t)x1;
+            const uint32_t o = __umul24((uint32_t)y1, (uint32_t)geo.pitch[1]) + (uint32_t)x1;
             emit(0, (uint32_t)geo.off[1][0] + o, kFieldX1 + j);
             emit(1, (uint32_t)geo.off[1][1] + o, kFieldY1 + j);
         }
@@ -819,7 +840,7 @@ VSHIP_INLINE void store_tile(const FrontJob& job, const PairGeom& geo, int tile_
     if (t < 128) {
         const int x2 = tile_x * 16 + (t & 15), y2 = tile_y * 8 + (t >> 4);
         if (x2 < geo.w[2] && y2 < geo.h[2]) {
-            const uint32_t o = (uint32_t)y2 * (uint32_t)geo.pitch[2] + (uint32_t)x2;
+            const uint32_t o = __umul24((uint32_t)y2, (uint32_t)geo.pitch[2]) + (uint32_t)x2;
             emit(0, (uint32_t)geo.off[2][0] + o, kFieldX2);
             emit(1, (uint32_t)geo.off[2][1] + o, kFieldY2);
         }
@@ -831,14 +852,14 @@ VSHIP_INLINE void store_tile(const FrontJob& job, const PairGeom& geo, int tile_
         if (lane < 16) {
             const int x3 = tile_x * 8 + (lane & 7), y3 = tile_y * 4 + 2 * w + (lane >> 3);
             if (x3 < geo.w[3] && y3 < geo.h[3]) {
-                const uint32_t o = (uint32_t)y3 * (uint32_t)geo.pitch[3] + (uint32_t)x3;
+                const uint32_t o = __umul24((uint32_t)y3, (uint32_t)geo.pitch[3]) + (uint32_t)x3;
 #pragma unroll
                 for (int c = 0; c < 3; c++) emit(c, (uint32_t)geo.off[3][c] + o, kFieldL3 + c);
             }
         } else if (lane < 20) {
             const int x4 = tile_x * 4 + (lane - 16), y4 = tile_y * 2 + w;
             if (x4 < geo.w[4] && y4 < geo.h[4]) {
-                const uint32_t o = (uint32_t)y4 * (uint32_t)geo.pitch[4] + (uint32_t)x4;
+                const uint32_t o = __umul24((uint32_t)y4, (uint32_t)geo.pitch[4]) + (uint32_t)x4;
 #pragma unroll
                 for (int c = 0; c < 3; c++) emit(c, (uint32_t)geo.off[4][c] + o, kFieldL3 + c);
             }
@@ -919,7 +940,7 @@ VSHIP_INLINE void frontend_tile(const FrontJob& job, const FrontColor* __restric
                 compute_tile(job, k, colors[job.color[k]], geo, tile_x, tile_y, work, l1_area + k * kL1Floats, r, P);
             } else {
                 const FrontColor fc = front_color<SPEC>(colors[job.color[k]]);
-                compute_tile(job, k, fc, geo, tile_x, tile_y, work, l1_area + k * kL1Floats, r, P);
+                compute_tile<true>(job, k, fc, geo, tile_x, tile_y, work, l1_area + k * kL1Floats, r, P);
             }
             void* out = job.out[k];
             store_tile(
@@ -998,6 +1019,7 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
         float* work = lds + kFrontStashFloats;
         float* l1_area = work + kChromaLdsMax;
         const int t = threadIdx.x;
+        constexpr bool kO32 = SPEC != kFrontGeneric;  // layout instantiations: 32-bit sample offsets
         // descriptor of image color index i, with the layout's constants folded in
         auto color = [&](int i) {
             if constexpr (SPEC == kFrontGeneric) return colors[i];
@@ -1007,7 +1029,7 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
         if (item < end) {
             const FrontJob& job = jobs[front_job_of(jobs, njobs, item)];
             const int local = item - job.tile_start;
-            raw = raw_tile(job, 0, color(job.color[0]), local % job.tiles_x, local / job.tiles_x);
+            raw = raw_tile<kO32>(job, 0, color(job.color[0]), local % job.tiles_x, local / job.tiles_x);
         }
         for (; item < end; item += step) {
             const FrontJob& job = jobs[front_job_of(jobs, njobs, item)];
@@ -1019,8 +1041,8 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
             TileXyb a, b;
             float P[3][kBH][kBW];
             unpack_raw(c0, raw, work, P);
-            raw = raw_tile(job, 1, c1, tile_x, tile_y);  // in flight during image 0
-            compute_tile(job, 0, c0, geo, tile_x, tile_y, work, l1_area, a, P, true);
+            raw = raw_tile<kO32>(job, 1, c1, tile_x, tile_y);  // in flight during image 0
+            compute_tile<kO32>(job, 0, c0, geo, tile_x, tile_y, work, l1_area, a, P, true);
 #pragma unroll
             for (int f = 0; f < kFields; f++) stash[f * kFrontThreads + t] = field(a, f);
             unpack_raw(c1, raw, work, P);
@@ -1028,9 +1050,9 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
             if (next < end) {  // the next tile's image 0, in flight during image 1
                 const FrontJob& nj = jobs[front_job_of(jobs, njobs, next)];
                 const int nl = next - nj.tile_start;
-                raw = raw_tile(nj, 0, color(nj.color[0]), nl % nj.tiles_x, nl / nj.tiles_x);
+                raw = raw_tile<kO32>(nj, 0, color(nj.color[0]), nl % nj.tiles_x, nl / nj.tiles_x);
             }
-            compute_tile(job, 1, c1, geo, tile_x, tile_y, work, l1_area + kL1Floats, b, P, true);
+            compute_tile<kO32>(job, 1, c1, geo, tile_x, tile_y, work, l1_area + kL1Floats, b, P, true);
             void* o0 = job.out[0];
             void* o1 = job.out[1];
             store_tile(
@@ -1164,6 +1186,18 @@ int front_spec(const FrontColor& a, const FrontColor& b)
     if (a.sample == kSampleU8) return kFrontU8Yuv420Bt709;
     if (a.sample == kSampleU16) return kFrontU16Yuv420Bt709;
     return kFrontGeneric;
+}
+
+bool front_offsets_fit(const FrontJob& job, const FrontColor& c0, const FrontColor& c1)
+{
+    const FrontColor* c[2] = {&c0, &c1};
+    for (int k = 0; k < 2; k++)
+        for (int p = 0; p < 3; p++) {
+            const int64_t s = job.stride[k][p];
+            const int64_t rows = p == 0 ? job.height : ((job.height - 1) >> c[k]->subh) + 1;
+            if (s <= 0 || s >= (int64_t(1) << 24) || s * rows > int64_t(0xFFFFFFFF)) return false;
+        }
+    return true;
 }
 
 template <typename StoreT>

@@ -20,6 +20,10 @@ enum FrontSpec : int {
 };
 // The instantiation serving a pair of images with these descriptors.
 int front_spec(const FrontColor& a, const FrontColor& b);
+// Whether every sample row of the job's six planes starts within 2^32 bytes
+// of its plane's base with a stride below 2^24 (the layout instantiations
+// address samples with 32-bit offsets; callers use kFrontGeneric otherwise).
+bool front_offsets_fit(const FrontJob& job, const FrontColor& c0, const FrontColor& c1);
 
 // counts: device array indexed by CountSlot. With fewer workgroups than items
 // the kernels loop over items (persistent mode).
