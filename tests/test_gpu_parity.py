@@ -153,6 +153,43 @@ def test_device_entry_matches_host_entry():
     assert host == dev
 
 
+def test_line_sizes_beyond_32bit_sample_offsets():
+    """Line sizes of 2^24 bytes leave the 32-bit sample offsets of the
+    layout-specific front-end; such calls run the generic front-end, and the
+    score equals that of the same pair with compact rows (single-pair handler
+    and a batch that mixes both)."""
+    cs = S.colorspace(40, 30)
+    ref, dist, strides = cases.perturbed_pair(cs, seed=23, pad=0)
+    big = [1 << 24] * 3
+    bufs, dptr, dptr_big = [], [[], []], [[], []]
+    for k, planes in enumerate((ref, dist)):
+        for p in planes:
+            b = S.DeviceBuffer(p.nbytes)
+            b.upload(p)
+            bufs.append(b)
+            dptr[k].append(b.address)
+            rows = p.reshape(p.shape[0], -1)
+            bb = S.DeviceBuffer((rows.shape[0] - 1) * big[0] + rows.shape[1])
+            for r in range(rows.shape[0]):
+                bb.upload(rows[r], offset=r * big[0])
+            bufs.append(bb)
+            dptr_big[k].append(bb.address)
+    try:
+        with S.SSIMU2Handler(cs, cs) as hd:
+            compact = hd.compute_device(dptr[0], dptr[1], strides, strides)
+            wide = hd.compute_device(dptr_big[0], dptr_big[1], big, big)
+        with S.SSIMU2Batch(2, S.STORAGE_FP32) as b:
+            b.set_tuning(7, 2, 2)
+            both = b.compute([cs, cs], [cs, cs], [dptr[0], dptr_big[0]], [dptr[1], dptr_big[1]], [strides, big],
+                             [strides, big])
+    finally:
+        for b in bufs:
+            b.free()
+    _say("wide rows", compact, wide, both)
+    assert wide == compact
+    assert both[0] == both[1] == compact
+
+
 def test_mixed_colourspaces_in_one_pair():
     w, h = 120, 80
     cs1 = S.colorspace(w, h)
