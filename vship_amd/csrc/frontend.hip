@@ -450,6 +450,48 @@ VSHIP_INLINE float dot5(const float w[5], const float v[5])
     return fmaf(w[0], v[0], fmaf(w[1], v[1], fmaf(w[2], v[2], fmaf(w[3], v[3], w[4] * v[4]))));
 }
 
+// 4:2:0 with chroma horizontally co-sited and vertically centred (the layout
+// instantiations): horizontal phase 0 copies the sample (weights 0 0 1 0 0),
+// horizontal phase 1 and vertical phase 1 have w[0] = 0, vertical phase 0 has
+// w[4] = 0. Dropping those terms from dot5's chain gives the same floats
+// (fmaf(0, v, x) == x and fmaf(w, v, 0 * u) == w * v for finite samples).
+// The horizontal items (2 planes x 20 staged rows, 32 column pairs each) go
+// to the threads in a loop of fixed trip count.
+VSHIP_INLINE void chroma_420_left(const FrontColor& fc, const float* s_chroma, float* s_mid, float P[3][kBH][kBW])
+{
+    constexpr int ccols = (kFrontTileW >> 1) + 4, crows = (kFrontTileH >> 1) + 4;
+    constexpr int plane_elems = ccols * crows;
+    constexpr int kRowStep = kFrontThreads / 32;
+    constexpr int kIters = (2 * crows + kRowStep - 1) / kRowStep;
+    const int t = threadIdx.x;
+    const int tx = t % kTX, ty = t / kTX;
+    const int m = t & 31;
+#pragma unroll 1
+    for (int it = 0; it < kIters; it++) {
+        const int e = (t >> 5) + kRowStep * it;
+        if (e >= 2 * crows) break;
+        const int p = e >= crows ? 1 : 0, r = e - p * crows;
+        const float* src = s_chroma + p * plane_elems + r * ccols + m;
+        const float v1 = src[1], v2 = src[2], v3 = src[3], v4 = src[4];
+        const float h1 = fmaf(fc.hw[1][1], v1, fmaf(fc.hw[1][2], v2, fmaf(fc.hw[1][3], v3, fc.hw[1][4] * v4)));
+        *reinterpret_cast<float2*>(s_mid + (p * crows + r) * kMidPitch + 2 * m) = make_float2(v2, h1);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+        float2 v2[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++)
+            v2[q] = *reinterpret_cast<const float2*>(s_mid + (p * crows + ty + q) * kMidPitch + 2 * tx);
+        const float* w0 = fc.vw[0];
+        const float* w1 = fc.vw[1];
+        P[1 + p][0][0] = fmaf(w0[0], v2[0].x, fmaf(w0[1], v2[1].x, fmaf(w0[2], v2[2].x, w0[3] * v2[3].x)));
+        P[1 + p][0][1] = fmaf(w0[0], v2[0].y, fmaf(w0[1], v2[1].y, fmaf(w0[2], v2[2].y, w0[3] * v2[3].y)));
+        P[1 + p][1][0] = fmaf(w1[1], v2[1].x, fmaf(w1[2], v2[2].x, fmaf(w1[3], v2[3].x, w1[4] * v2[4].x)));
+        P[1 + p][1][1] = fmaf(w1[1], v2[1].y, fmaf(w1[2], v2[2].y, fmaf(w1[3], v2[3].y, w1[4] * v2[4].y)));
+    }
+}
+
 VSHIP_INLINE void chroma_2x(const FrontColor& fc, const float* s_chroma, float* s_mid, int ccols, int crows,
                             bool both, float P[3][kBH][kBW])
 {
@@ -518,10 +560,11 @@ VSHIP_INLINE void chroma_2x(const FrontColor& fc, const float* s_chroma, float* 
 // preloaded: P's plane values and the staged chroma samples are already in
 // place (unpack_raw of samples loaded ahead by raw_tile); otherwise they are
 // loaded here.
-template <bool O32 = false>
+template <int SPEC = kFrontGeneric>
 VSHIP_INLINE void to_linear(const FrontJob& job, int k, const FrontColor& fc, int tile_x, int tile_y, float* lds,
                             float P[3][kBH][kBW], bool preloaded = false)
 {
+    constexpr bool O32 = SPEC != kFrontGeneric;  // 32-bit sample offsets (front_offsets_fit)
     const uint8_t* const* plane = job.plane[k];
     const int64_t* pstride = job.stride[k];
     const int t = threadIdx.x;
@@ -596,7 +639,9 @@ VSHIP_INLINE void to_linear(const FrontJob& job, int k, const FrontColor& fc, in
         float* s_mid = lds + 2 * plane_elems;
         __syncthreads();
         const bool both = fc.subw && fc.subh;
-        if (fc.subw <= 1 && fc.subh <= 1) {
+        if constexpr (SPEC != kFrontGeneric) {
+            chroma_420_left(fc, s_chroma, s_mid, P);
+        } else if (fc.subw <= 1 && fc.subh <= 1) {
             chroma_2x(fc, s_chroma, s_mid, ccols, crows, both, P);
         } else if (both) {
             // horizontal pass over every staged chroma row first, like the
@@ -715,7 +760,7 @@ VSHIP_INLINE void down_pairs(float v[3], bool x_out, bool y_out, FX fx, FY fy)
 }
 
 // P holds the image's unpacked plane values when preloaded (see to_linear).
-template <bool O32 = false>
+template <int SPEC = kFrontGeneric>
 VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc, const PairGeom& geo, int tile_x,
                                int tile_y, float* lds, float* lds_l1, TileXyb& res, float P[3][kBH][kBW],
                                bool preloaded = false)
@@ -725,7 +770,7 @@ VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc,
     const int W = job.width, H = job.height;
     const int x0 = tile_x * kFrontTileW + kBW * tx;
     const int y0 = tile_y * kFrontTileH + kBH * ty;
-    to_linear<O32>(job, k, fc, tile_x, tile_y, lds, P, preloaded);
+    to_linear<SPEC>(job, k, fc, tile_x, tile_y, lds, P, preloaded);
 
     // ---- level 0: Y only
 #pragma unroll
@@ -899,11 +944,11 @@ VSHIP_INLINE void gstore2<__half>(void* base, uint32_t off, float v0, float v1)
         f16x2_t{(_Float16)v0, (_Float16)v1};
 }
 
-// Colour layouts with their own instantiation of the single-pair front-end:
-// with the sample type, subsampling, transfer and matrix steps known at
-// compile time the per-format switches disappear and the kernel needs fewer
-// registers. The descriptor's values (ranges, matrices, weights) stay
-// run-time.
+// Colour layouts with their own instantiation of the front-end: with the
+// sample type, subsampling, chroma siting, transfer and matrix steps known at
+// compile time the per-format switches disappear, the chroma pass drops its
+// zero taps and the kernel needs fewer registers. The descriptor's values
+// (ranges, matrices, weights) stay run-time.
 template <int SPEC>
 VSHIP_INLINE FrontColor front_color(const FrontColor& in)
 {
@@ -940,7 +985,7 @@ VSHIP_INLINE void frontend_tile(const FrontJob& job, const FrontColor* __restric
                 compute_tile(job, k, colors[job.color[k]], geo, tile_x, tile_y, work, l1_area + k * kL1Floats, r, P);
             } else {
                 const FrontColor fc = front_color<SPEC>(colors[job.color[k]]);
-                compute_tile<true>(job, k, fc, geo, tile_x, tile_y, work, l1_area + k * kL1Floats, r, P);
+                compute_tile<SPEC>(job, k, fc, geo, tile_x, tile_y, work, l1_area + k * kL1Floats, r, P);
             }
             void* out = job.out[k];
             store_tile(
@@ -1042,7 +1087,7 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
             float P[3][kBH][kBW];
             unpack_raw(c0, raw, work, P);
             raw = raw_tile<kO32>(job, 1, c1, tile_x, tile_y);  // in flight during image 0
-            compute_tile<kO32>(job, 0, c0, geo, tile_x, tile_y, work, l1_area, a, P, true);
+            compute_tile<SPEC>(job, 0, c0, geo, tile_x, tile_y, work, l1_area, a, P, true);
 #pragma unroll
             for (int f = 0; f < kFields; f++) stash[f * kFrontThreads + t] = field(a, f);
             unpack_raw(c1, raw, work, P);
@@ -1052,7 +1097,7 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
                 const int nl = next - nj.tile_start;
                 raw = raw_tile<kO32>(nj, 0, color(nj.color[0]), nl % nj.tiles_x, nl / nj.tiles_x);
             }
-            compute_tile<kO32>(job, 1, c1, geo, tile_x, tile_y, work, l1_area + kL1Floats, b, P, true);
+            compute_tile<SPEC>(job, 1, c1, geo, tile_x, tile_y, work, l1_area + kL1Floats, b, P, true);
             void* o0 = job.out[0];
             void* o1 = job.out[1];
             store_tile(
@@ -1180,7 +1225,8 @@ hipError_t launch_frontend(const FrontJob* jobs, const int* counts, int grid, co
 int front_spec(const FrontColor& a, const FrontColor& b)
 {
     auto fixed = [](const FrontColor& c) {
-        return c.subw == 1 && c.subh == 1 && c.transfer == 1 && c.use_pre == 1 && c.use_post == 0;
+        return c.subw == 1 && c.subh == 1 && c.h_cosited == 1 && c.v_cosited == 0 && c.transfer == 1 &&
+               c.use_pre == 1 && c.use_post == 0;
     };
     if (!fixed(a) || !fixed(b) || a.sample != b.sample) return kFrontGeneric;
     if (a.sample == kSampleU8) return kFrontU8Yuv420Bt709;

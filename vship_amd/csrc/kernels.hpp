@@ -11,8 +11,8 @@ size_t frontend_lds_bytes(const FrontColor& fc);
 size_t frontend_max_lds_bytes();
 
 // Colour layouts with a front-end instantiation of their own (8- and 16-bit
-// YUV 4:2:0, BT.709 transfer, no linear-light matrix); kFrontGeneric
-// handles every layout.
+// YUV 4:2:0 with left-sited chroma, BT.709 transfer, no linear-light
+// matrix); kFrontGeneric handles every layout.
 enum FrontSpec : int {
     kFrontGeneric = 0,
     kFrontU8Yuv420Bt709 = 1,
