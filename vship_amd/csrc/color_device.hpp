@@ -142,9 +142,15 @@ __device__ inline void mat3_apply(const float m[9], float& x, float& y, float& z
     x = a; y = b; z = c;
 }
 
-// Linear RGB -> positive XYB, reference src/ssimu2/makeXYB.hpp:6-52. The
-// reference adds the last three offsets in double (:44-46); float additions
-// here differ by at most one ulp of the stored value.
+// Linear RGB -> positive XYB, reference src/ssimu2/makeXYB.hpp:6-52, with
+// the cube-root offset c (-0.15595...) folded into the constants: for cube
+// roots l, m, s of the three opsin mixes,
+//   X' = 14 * ((l + c) - (m + c)) / 2 + 0.42 = 7 (l - m) + 0.42
+//   Y' = ((l + c) + (m + c)) / 2 + 0.01      = (l + m) / 2 + (c + 0.01)
+//   B' = (s + c) - (Y' - 0.01) + 0.55        = s - (l + m) / 2 + 0.55
+// (the offset cancels in X' and B'). Fewer roundings than the reference's
+// float steps (which add the last three offsets in double, :44-46): the
+// stored values differ from it by about one ulp.
 struct XYB {
     float x, y, b;
 };
@@ -152,20 +158,16 @@ struct XYB {
 __device__ inline XYB to_xyb(float r, float g, float b, bool need_b)
 {
     const float bias = 0.0037930734f;
-    const float cbias = -0.1559542025327239f;
-    float l = fmaf(0.30f, r, fmaf(0.622f, g, fmaf(0.078f, b, bias)));
-    float m = fmaf(0.23f, r, fmaf(0.692f, g, fmaf(0.078f, b, bias)));
-    l = cbrt_pos(l) + cbias;
-    m = cbrt_pos(m) + cbias;
+    constexpr float kYOffset = -0.1559542025327239f + 0.01f;
+    const float l = cbrt_pos(fmaf(0.30f, r, fmaf(0.622f, g, fmaf(0.078f, b, bias))));
+    const float m = cbrt_pos(fmaf(0.23f, r, fmaf(0.692f, g, fmaf(0.078f, b, bias))));
+    const float h = l + m;
     XYB o;
-    const float X = 0.5f * (l - m);
-    const float Y = X + m;
-    o.x = fmaf(X, 14.0f, 0.42f);
-    o.y = Y + 0.01f;
+    o.x = fmaf(l - m, 7.0f, 0.42f);
+    o.y = fmaf(h, 0.5f, kYOffset);
     if (need_b) {
-        float s = fmaf(0.24342269f, r, fmaf(0.20476745f, g, fmaf(0.55180986f, b, bias)));
-        s = cbrt_pos(s) + cbias;
-        o.b = (s - Y) + 0.55f;
+        const float s = cbrt_pos(fmaf(0.24342269f, r, fmaf(0.20476745f, g, fmaf(0.55180986f, b, bias))));
+        o.b = fmaf(h, -0.5f, s) + 0.55f;
     } else {
         o.b = 0.0f;
     }
