@@ -5,6 +5,12 @@ python -m vship_amd.profile_summary trace RESULTS_DB
 python -m vship_amd.profile_summary pmc COUNTER KERNEL_DB COPY_DB COPY_MIB
     mean KiB per dispatch of COUNTER for every kernel, and the same counter
     scaled by what it reads for a plain copy kernel of COPY_MIB MiB
+python -m vship_amd.profile_summary valu KERNEL_DB SIMDS XCDS
+    VALU busy fraction per kernel from a run that collected SQ_ACTIVE_INST_VALU,
+    SQ_INSTS_VALU and GRBM_GUI_ACTIVE: SQ_ACTIVE_INST_VALU counts quad-cycles
+    (a wave64 VALU instruction holds a 16-lane SIMD for 4 cycles) summed over
+    the chip, GRBM_GUI_ACTIVE counts the dispatch's cycles once per XCD, so
+    busy = 4 * SQ_ACTIVE_INST_VALU / (SIMDS * GRBM_GUI_ACTIVE / XCDS)
 """
 from __future__ import annotations
 
@@ -49,6 +55,17 @@ def main(argv=None) -> int:
         print(f"{counter}: copy kernel of {mib:g} MiB reads {copy:.1f} KiB per dispatch -> scale {scale:.4f}")
         for k, v in sorted(pmc(counter, db).items()):
             print(f"| `{k}` | {v:.1f} | {v * scale * 1024:.0f} |")
+        return 0
+    if a and a[0] == "valu":
+        db, simds, xcds = a[1], float(a[2]), float(a[3])
+        act, ins, grbm = pmc("SQ_ACTIVE_INST_VALU", db), pmc("SQ_INSTS_VALU", db), pmc("GRBM_GUI_ACTIVE", db)
+        print("| kernel | VALU instructions | SQ_ACTIVE_INST_VALU | cycles | VALU busy |")
+        print("|---|---|---|---|---|")
+        for k in sorted(act):
+            if k not in grbm or k not in ins or grbm[k] <= 0:
+                continue
+            cyc = grbm[k] / xcds
+            print(f"| `{k}` | {ins[k]:.0f} | {act[k]:.0f} | {cyc:.0f} | {4.0 * act[k] / (simds * cyc):.3f} |")
         return 0
     print(__doc__)
     return 2
