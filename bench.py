@@ -295,10 +295,12 @@ def main():
     in_bytes = sum(p.nbytes for p in frames[0][0])
     front_bytes = 2.0 * in_bytes + 2.0 * 4.0 * a_px
     blur_bytes = 2.0 * 4.0 * a_px
-    traffic = {}
+    traffic, valu_busy = {}, {}
     traffic_file = os.path.join(ROOT, "profiles", "kernel_traffic_1080p.json")
     if os.path.exists(traffic_file):
-        traffic = json.load(open(traffic_file)).get("bytes_per_launch", {})
+        counters = json.load(open(traffic_file))
+        traffic = counters.get("bytes_per_launch", {})
+        valu_busy = counters.get("valu_busy_batched_fp16", {})
     per_kernel = {
         FRONT_KERNEL: {
             "ms": round(ms_front, 4), "algorithmic_bytes": front_bytes,
@@ -319,6 +321,9 @@ def main():
     roofline = {
         "kernel": dominant, "bound": "hbm", "achieved": d["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": d["frac"], "traffic": d["traffic"], "per_kernel": per_kernel, "path_bytes_per_pair": bytes_pair,
+        # what bounds the kernels instead: vector-ALU issue (rocprofv3 counters
+        # of the same kernels on the batched FP16 path, 16 pairs per launch)
+        "limiter": "valu", "valu_busy": {k: v for k, v in valu_busy.items() if k != "source"},
     }
 
     # batched FP16 storage path on the same 240 pairs: one graph replay per step
