@@ -976,22 +976,39 @@ VSHIP_INLINE void frontend_tile(const FrontJob& job, const FrontColor* __restric
     float* work = lds + (kHalf ? kFrontStashFloats : 0);
     float* l1_area = work + kChromaLdsMax;  // [2 images][kL1Floats]
     const int t = threadIdx.x;
-    if constexpr (!kHalf) {
+    if constexpr (!kHalf && SPEC == kFrontGeneric) {
 #pragma unroll 1
         for (int k = 0; k < 2; k++) {
             TileXyb r;
             float P[3][kBH][kBW];
-            if constexpr (SPEC == kFrontGeneric) {
-                compute_tile(job, k, colors[job.color[k]], geo, tile_x, tile_y, work, l1_area + k * kL1Floats, r, P);
-            } else {
-                const FrontColor fc = front_color<SPEC>(colors[job.color[k]]);
-                compute_tile<SPEC>(job, k, fc, geo, tile_x, tile_y, work, l1_area + k * kL1Floats, r, P);
-            }
+            compute_tile(job, k, colors[job.color[k]], geo, tile_x, tile_y, work, l1_area + k * kL1Floats, r, P);
             void* out = job.out[k];
             store_tile(
                 job, geo, tile_x, tile_y, [&](int, uint32_t off, int f) { gstore1<StoreT>(out, off, field(r, f)); },
                 [&](int, uint32_t off, int f) { gstore2<StoreT>(out, off, field(r, f), field(r, f + 1)); });
         }
+    } else if constexpr (!kHalf) {
+        // layout instantiation: image 1's raw samples load while image 0 is
+        // computed (the chroma staging area is free again once compute_tile
+        // has passed its level-2 barrier)
+        TileXyb r;
+        float P[3][kBH][kBW];
+        const FrontColor c0 = front_color<SPEC>(colors[job.color[0]]);
+        const FrontColor c1 = front_color<SPEC>(colors[job.color[1]]);
+        RawTile raw = raw_tile<true>(job, 0, c0, tile_x, tile_y);
+        unpack_raw(c0, raw, work, P);
+        raw = raw_tile<true>(job, 1, c1, tile_x, tile_y);
+        compute_tile<SPEC>(job, 0, c0, geo, tile_x, tile_y, work, l1_area, r, P, true);
+        void* o0 = job.out[0];
+        store_tile(
+            job, geo, tile_x, tile_y, [&](int, uint32_t off, int f) { gstore1<StoreT>(o0, off, field(r, f)); },
+            [&](int, uint32_t off, int f) { gstore2<StoreT>(o0, off, field(r, f), field(r, f + 1)); });
+        unpack_raw(c1, raw, work, P);
+        compute_tile<SPEC>(job, 1, c1, geo, tile_x, tile_y, work, l1_area + kL1Floats, r, P, true);
+        void* o1 = job.out[1];
+        store_tile(
+            job, geo, tile_x, tile_y, [&](int, uint32_t off, int f) { gstore1<StoreT>(o1, off, field(r, f)); },
+            [&](int, uint32_t off, int f) { gstore2<StoreT>(o1, off, field(r, f), field(r, f + 1)); });
     } else {
         TileXyb a, b;
         float P[3][kBH][kBW];
@@ -1024,8 +1041,9 @@ VSHIP_INLINE void frontend_tile(const FrontJob& job, const FrontColor* __restric
 // FP16 layout: the raw samples of each image are loaded while the previous
 // image is computed (image 1 of a tile during image 0, image 0 of the next
 // tile during image 1), so the memory latency of a tile's inputs overlaps
-// work. The FP32 instantiation has no registers to spare for the 12 words in
-// flight and loads each image when it starts.
+// work. The generic FP32 instantiation has no registers to spare for the 12
+// words in flight and loads each image when it starts; the FP32 layout
+// instantiations load image 1 of a tile during image 0 (frontend_tile).
 VSHIP_INLINE int front_job_of(const FrontJob* __restrict__ jobs, int njobs, int item)
 {
     // largest j with tile_start <= item
