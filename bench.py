@@ -258,8 +258,8 @@ def main():
     ap.add_argument("--no-screening", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; frame pairs shard across GPUs with one "
-                                   "handler per device and no collectives, see DESIGN.md"}))
+        print(json.dumps({"error": "bench.py measures one GPU; frame pairs shard across GPUs with one "
+                                   "handler or batch context per device and no collectives"}))
         sys.exit(2)
 
     t0 = time.perf_counter()
