@@ -711,7 +711,23 @@ VSHIP_INLINE void to_linear(const FrontJob& job, int k, const FrontColor& fc, in
     }
 
     // matrix, EOTF, linear-light matrix with primaries (P becomes linear RGB)
-    if (fc.use_pre) {
+    if constexpr (SPEC != kFrontGeneric) {
+        // YCbCr matrix with unit luma column and R(Cb) = B(Cr) = 0 (front_spec
+        // checks): mat3_apply's chains without the terms fmaf(0, v, t) == t
+        // and fmaf(1, Y, t) == Y + t, so the same floats (no contraction of
+        // the product into the sum)
+#pragma clang fp contract(off)
+        const float* m = fc.pre;
+#pragma unroll
+        for (int j = 0; j < kBH; j++)
+#pragma unroll
+            for (int i = 0; i < kBW; i++) {
+                const float y = P[0][j][i], u = P[1][j][i], v = P[2][j][i];
+                P[0][j][i] = y + m[2] * v;
+                P[1][j][i] = y + fmaf(m[4], u, m[5] * v);
+                P[2][j][i] = y + m[7] * u;
+            }
+    } else if (fc.use_pre) {
 #pragma unroll
         for (int j = 0; j < kBH; j++)
 #pragma unroll
@@ -1244,7 +1260,8 @@ int front_spec(const FrontColor& a, const FrontColor& b)
 {
     auto fixed = [](const FrontColor& c) {
         return c.subw == 1 && c.subh == 1 && c.h_cosited == 1 && c.v_cosited == 0 && c.transfer == 1 &&
-               c.use_pre == 1 && c.use_post == 0;
+               c.use_pre == 1 && c.use_post == 0 && c.pre[0] == 1.0f && c.pre[1] == 0.0f && c.pre[3] == 1.0f &&
+               c.pre[6] == 1.0f && c.pre[8] == 0.0f;
     };
     if (!fixed(a) || !fixed(b) || a.sample != b.sample) return kFrontGeneric;
     if (a.sample == kSampleU8) return kFrontU8Yuv420Bt709;
