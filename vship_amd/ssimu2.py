@@ -389,6 +389,18 @@ class SSIMU2Batch:
                                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
 
+    def prepare(self, src, dis, planes1, planes2, strides1, strides2):
+        """ctypes arguments of one batched call, built once for repeated calls on the same planes (as a
+        C caller holds its descriptor and pointer arrays); see compute_prepared."""
+        return self._args(src, dis, planes1, planes2, strides1, strides2)
+
+    def compute_prepared(self, args, out: np.ndarray) -> np.ndarray:
+        """VshipAMD_SSIMU2BatchCompute on arguments from prepare; scores into out (float64, >= n)."""
+        n, cs1, cs2, p1, p2, l1, l2 = args
+        check(self._lib.VshipAMD_SSIMU2BatchCompute(self.handle, n, cs1, cs2, p1, p2, l1, l2,
+                                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
     def autotune(self, src, dis, planes1, planes2, strides1, strides2, reps: int = 2):
         n, cs1, cs2, p1, p2, l1, l2 = self._args(src, dis, planes1, planes2, strides1, strides2)
         t = BatchTuning()
