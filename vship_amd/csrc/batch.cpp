@@ -71,6 +71,24 @@ Ssimu2Batch::~Ssimu2Batch()
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
+const ImageSpec& Ssimu2Batch::cached_spec(const Vship_Colorspace_t& cs)
+{
+    std::string key(reinterpret_cast<const char*>(&cs), sizeof(cs));
+    auto it = spec_cache_.find(key);
+    if (it != spec_cache_.end()) return it->second;
+    if (spec_cache_.size() > 256) spec_cache_.clear();
+    return spec_cache_.emplace(std::move(key), make_image_spec(cs)).first->second;
+}
+
+const PairLayout& Ssimu2Batch::cached_layout(int w, int h, int blur_config)
+{
+    const auto key = std::make_tuple(w, h, blur_config);
+    auto it = layout_cache_.find(key);
+    if (it != layout_cache_.end()) return it->second;
+    if (layout_cache_.size() > 256) layout_cache_.clear();
+    return layout_cache_.emplace(key, make_pair_layout(w, h, blur_config)).first->second;
+}
+
 void Ssimu2Batch::build_tables(const BatchInput& in)
 {
     if (in.npairs <= 0 || in.npairs > max_pairs_) VSHIP_THROW(Vship_BadPointer, "npairs outside [1, max_pairs]");
@@ -84,11 +102,9 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
     int64_t elems = 0;
     int ftiles = 0, btiles = 0;
     double in_bytes = 0.0, active = 0.0, flops = 0.0, blocks = 0.0;
-    std::vector<PairLayout> layouts;
-    layouts.reserve(in.npairs);
     for (int i = 0; i < in.npairs; i++) {
-        const ImageSpec s1 = make_image_spec(in.src[i]);
-        const ImageSpec s2 = make_image_spec(in.dis[i]);
+        const ImageSpec s1 = cached_spec(in.src[i]);
+        const ImageSpec s2 = cached_spec(in.dis[i]);
         if (s1.resample || s2.resample)
             VSHIP_THROW(Vship_NonRGBSInput, "crop and resize are supported by the single-pair handler only");
         if (s1.width != s2.width || s1.height != s2.height)
@@ -103,8 +119,7 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
                     VSHIP_THROW(Vship_BadPointer, "lineSize smaller than one row of samples");
                 in_bytes += (double)specs[k]->plane_w[p] * specs[k]->plane_h[p] * specs[k]->bytes;
             }
-        layouts.push_back(make_pair_layout((int)s1.width, (int)s1.height, tuning_.blur_config));
-        const PairLayout& L = layouts.back();
+        const PairLayout& L = cached_layout((int)s1.width, (int)s1.height, tuning_.blur_config);
         geoms_.push_back(L.geom);
         FrontJob j;
         std::memset(&j, 0, sizeof(j));

@@ -18,6 +18,10 @@
 #include <stdint.h>
 #include <vector>
 
+#include <map>
+#include <string>
+#include <tuple>
+
 #include "../../include/VshipAPI.h"
 #include "engine.hpp"
 #include "ssimu2_common.hpp"
@@ -78,6 +82,13 @@ private:
     std::vector<MapJob> maps_;
     std::vector<ScoreJob> sjobs_;
     std::vector<int64_t> xyb_off_;   // per image (2 per pair): element offset in the arena
+    // descriptors and layouts of earlier calls (a batch usually repeats a few
+    // colourspaces and sizes): keyed by the descriptor's bytes and by
+    // (width, height, blur config)
+    std::map<std::string, ImageSpec> spec_cache_;
+    std::map<std::tuple<int, int, int>, PairLayout> layout_cache_;
+    const ImageSpec& cached_spec(const Vship_Colorspace_t& cs);
+    const PairLayout& cached_layout(int w, int h, int blur_config);
     int64_t xyb_elems_ = 0;
     int front_tiles_ = 0, blur_tiles_ = 0, npairs_ = 0;
     double work_bytes_ = 0.0, work_flops_ = 0.0;
