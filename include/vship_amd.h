@@ -35,8 +35,10 @@ EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2Features(Vship_SSIMU2Handler han
 EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2DebugPlane(Vship_SSIMU2Handler handler, int image, int plane,
                                                            int scale, float* out, int64_t* width, int64_t* height);
 
-/* Replays the kernels of the last compute `iters` times on the handler's
- * stream and returns the mean device time of each kernel (HIP events). */
+/* Replays the last compute `iters` times on the handler's stream and returns
+ * the mean device time of the front-end and blur/map kernels (HIP events) and
+ * the mean host time of the score's FP64 completion from the blur's partial
+ * sums (ms_score). */
 EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2Profile(Vship_SSIMU2Handler handler, int iters, double* ms_frontend,
                                                         double* ms_blur, double* ms_score);
 

@@ -64,8 +64,11 @@ HANDLER_BLUR_CONFIG = 7  # the single-pair handler's blur items: 64x128 strips
 
 def test_fp32_batch_equals_plain_path(pairs):
     """With the handler's blur item shape, FP32 storage reproduces the plain
-    path bit for bit (same kernels, same item partials, same reduction order);
-    the batch default (64x256 strips) differs only by the per-item centring."""
+    path's 108 features bit for bit (same kernels, same item partials, same
+    reduction order; the handler completes them on the host, the batch on the
+    GPU) and its score to the last bits of the final power (the host's and the
+    GPU's pow may differ by an ulp); the batch default (64x256 strips) differs
+    only by the per-item centring."""
     with S.SSIMU2Batch(16, S.STORAGE_FP32) as b:
         default = b.compute(*pairs.args())
         b.set_tuning(HANDLER_BLUR_CONFIG, 3, 4)
@@ -76,8 +79,8 @@ def test_fp32_batch_equals_plain_path(pairs):
             want = h.compute(pairs.host[i][0], pairs.host[i][1], pairs.strides[i], pairs.strides[i])
             wf = h.features()
         _say("fp32 batch", cs.width, cs.height, got[i], want, default[i])
-        assert got[i] == want
         np.testing.assert_array_equal(feats[i], wf)
+        assert abs(got[i] - want) <= 1e-12
         assert abs(default[i] - want) <= 1e-4
 
 
@@ -108,8 +111,8 @@ def test_graph_replay_with_changing_compositions(pairs):
 def test_layout_instantiation_batch_vs_oracle(kind):
     """A batch whose images all share the 8- or 16-bit YUV 4:2:0 BT.709 layout
     runs the front-end instantiation compiled for that layout: FP32 storage
-    equals the single-pair handler bit for bit and is within 0.005 of the FP64
-    oracle; FP16 storage equals the generic front-end bit for bit (forced by
+    equals the single-pair handler (to the final power's last bits) and is
+    within 0.005 of the FP64 oracle; FP16 storage equals the generic front-end bit for bit (forced by
     adding a pair of another layout to the batch) and is within 0.01 of the oracle on
     images of 64x64 and more; identity stays 100."""
     sizes = [(131, 75), (64, 64), (200, 117), (17, 9), (1, 1), (320, 180), (40, 30)]
@@ -131,7 +134,7 @@ def test_layout_instantiation_batch_vs_oracle(kind):
             want, _ = oracle.ssimu2(cs_i, ref, dp.strides[i], cs_i, dist, dp.strides[i])
             _say(f"layout batch kind {kind} {cs_i.width}x{cs_i.height}: fp32 {got32[i]:.6f} plain {plain:.6f} "
                  f"fp16 {got16[i]:.6f} generic fp16 {generic16[i]:.6f} oracle {want:.6f}")
-            assert got32[i] == plain
+            assert abs(got32[i] - plain) <= 1e-12  # same features; the final pow runs on host vs GPU
             assert abs(got32[i] - want) <= 0.005
             assert got16[i] == generic16[i]  # same arithmetic, constants folded
             # FP16 storage rounds every stored XYB sample to 11 bits; images

@@ -2,6 +2,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -389,11 +390,10 @@ Ssimu2Handler::Ssimu2Handler(const Vship_Colorspace_t& src, const Vship_Colorspa
     dmalloc((void**)&d_colors_, 3 * sizeof(FrontColor));
     dmalloc((void**)&d_geom_, sizeof(PairGeom));
     dmalloc((void**)&d_maps_, layout_.maps.size() * sizeof(MapJob));
-    dmalloc((void**)&d_score_jobs_, sizeof(ScoreJob));
     dmalloc((void**)&d_counts_, kCountSlots * sizeof(int));
-    dmalloc((void**)&d_partials_, (size_t)layout_.blur_tiles * kPartialStride * sizeof(float));
-    dmalloc((void**)&d_features_, 108 * sizeof(double));
-    dmalloc((void**)&d_score_, sizeof(double));
+    if (hipHostMalloc((void**)&h_partials_, (size_t)layout_.blur_tiles * kPartialStride * sizeof(float),
+                      hipHostMallocDefault) != hipSuccess)
+        VSHIP_THROW(Vship_OutOfRAM, "hipHostMalloc of the partial sums failed in SSIMU2 init");
     if (hipHostMalloc((void**)&h_front_jobs_, sizeof(FrontJob), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&h_score_, sizeof(double), hipHostMallocDefault) != hipSuccess)
         VSHIP_THROW(Vship_OutOfRAM, "hipHostMalloc failed in SSIMU2 init");
@@ -406,7 +406,6 @@ Ssimu2Handler::Ssimu2Handler(const Vship_Colorspace_t& src, const Vship_Colorspa
         m.b = static_cast<const float*>(xyb_[1]) + off;
     }
     FrontColor colors[3] = {spec_[0].fc, spec_[1].fc, lin_fc};
-    ScoreJob sj = {0, (int32_t)maps.size(), {0, 0}};
     int counts[kCountSlots] = {};
     counts[kCountFrontTiles] = layout_.front_tiles_per_image;
     counts[kCountFrontJobs] = 1;
@@ -416,9 +415,7 @@ Ssimu2Handler::Ssimu2Handler(const Vship_Colorspace_t& src, const Vship_Colorspa
     VSHIP_HIP_CHECK(hipMemcpy(d_maps_, maps.data(), maps.size() * sizeof(MapJob), hipMemcpyHostToDevice));
     VSHIP_HIP_CHECK(hipMemcpy(d_colors_, colors, sizeof(colors), hipMemcpyHostToDevice));
     VSHIP_HIP_CHECK(hipMemcpy(d_geom_, &layout_.geom, sizeof(PairGeom), hipMemcpyHostToDevice));
-    VSHIP_HIP_CHECK(hipMemcpy(d_score_jobs_, &sj, sizeof(sj), hipMemcpyHostToDevice));
     VSHIP_HIP_CHECK(hipMemcpy(d_counts_, counts, sizeof(counts), hipMemcpyHostToDevice));
-    VSHIP_HIP_CHECK(hipMemset(d_features_, 0, 108 * sizeof(double)));
     FrontJob& j = *h_front_jobs_;
     std::memset(&j, 0, sizeof(j));
     for (int i = 0; i < 2; i++) {
@@ -482,13 +479,10 @@ Ssimu2Handler::~Ssimu2Handler()
     (void)hipFree(d_colors_);
     (void)hipFree(d_geom_);
     (void)hipFree(d_maps_);
-    (void)hipFree(d_score_jobs_);
     (void)hipFree(d_counts_);
-    (void)hipFree(d_partials_);
-    (void)hipFree(d_features_);
-    (void)hipFree(d_score_);
     (void)hipHostFree(h_front_jobs_);
     (void)hipHostFree(h_score_);
+    if (h_partials_) (void)hipHostFree(h_partials_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -516,6 +510,57 @@ int Ssimu2Handler::call_spec() const
     return front_offsets_fit(*h_front_jobs_, spec_[0].fc, spec_[1].fc) ? front_spec_ : kFrontGeneric;
 }
 
+// FP64 completion of one pair's score from its blur items' partial sums
+// (reference src/ssimu2/score.hpp:315-320,422-460), on the host, with the
+// operations and order of the GPU's score kernel: per map job, lane k of a
+// wave sums items k, k + 64, ... and the lanes combine by the xor butterfly of
+// its wave reductions (lane m's result is feature m's); the weighted sum takes
+// features k and k + 64 per lane, then the butterfly; the polynomial has no
+// contraction on either side. The features are therefore the same doubles
+// as the batched path's (whose score kernel runs this on the GPU).
+// Lane `lane`'s value after K stages (offsets 32, 16, ... of the 64-lane xor
+// butterfly v[i] += v[i ^ off]): the same additions as the wave reduction,
+// evaluated for one lane only.
+template <int K>
+static double butterfly_lane(const double v[64], int lane)
+{
+    if constexpr (K == 0) {
+        return v[lane];
+    } else {
+        constexpr int off = 64 >> K;
+        return butterfly_lane<K - 1>(v, lane) + butterfly_lane<K - 1>(v, lane ^ off);
+    }
+}
+
+static double host_score(const std::vector<MapJob>& maps, const float* partials, double feat[108])
+{
+#pragma clang fp contract(off)
+    static const double kWeights[108] = VSHIP_SSIMU2_WEIGHTS;
+    for (int i = 0; i < 108; i++) feat[i] = 0.0;
+    for (const MapJob& job : maps) {
+        double acc[6][64] = {};
+        for (int k = 0; k < job.ntiles; k++) {
+            const float* p = partials + (size_t)(job.tile_start + k) * kPartialStride;
+            for (int m = 0; m < 6; m++) acc[m][k & 63] += (double)p[m];
+        }
+        const double n = (double)job.w * (double)job.h;
+        for (int m = 0; m < 3; m++) {
+            const int flag = m == 0 ? kMapSsim : (m == 1 ? kMapArtifact : kMapDetail);
+            if (!(job.flags & flag)) continue;
+            feat[job.feature_base + m] = butterfly_lane<6>(acc[m], m) / n;
+            feat[job.feature_base + 3 + m] = std::sqrt(std::sqrt(butterfly_lane<6>(acc[3 + m], m) / n));
+        }
+    }
+    double s[64];
+    for (int l = 0; l < 64; l++) {
+        s[l] = l < 108 ? kWeights[l] * feat[l] : 0.0;
+        if (l + 64 < 108) s[l] = std::fma(kWeights[l + 64], feat[l + 64], s[l]);
+    }
+    double x = butterfly_lane<6>(s, 0) * 0.9562382616834844;
+    x = (6.248496625763138e-5 * x * x) * x + 2.326765642916932 * x - 0.020884521182843837 * x * x;
+    return (x > 0.0) ? std::pow(x, 0.6276336467831387) * -10.0 + 100.0 : 100.0;
+}
+
 void Ssimu2Handler::launch_all()
 {
     if (d_lin_jobs_) {
@@ -538,10 +583,10 @@ void Ssimu2Handler::launch_all()
     // the job (plane pointers of this call) travels as a kernel argument
     VSHIP_HIP_CHECK(launch_frontend_single<float>(*h_front_jobs_, layout_.front_tiles_per_image, d_colors_, d_geom_,
                                                   call_spec(), stream_));
+    // the items' partial sums go straight to pinned host memory; run() forms
+    // the score from them (host_score), so a call is two kernel launches
     VSHIP_HIP_CHECK(launch_blur<float>(layout_.blur_config, d_maps_, d_counts_, blur_grid_for(layout_.blur_tiles),
-                                       taps_, d_partials_, stream_));
-    // the score goes straight to the pinned host word the caller reads
-    VSHIP_HIP_CHECK(launch_score(d_score_jobs_, d_counts_, 1, d_maps_, d_partials_, d_features_, h_score_, stream_));
+                                       taps_, h_partials_, stream_));
 }
 
 void Ssimu2Handler::run(const uint8_t* const p1[3], const uint8_t* const p2[3], const int64_t ls1[3],
@@ -561,6 +606,7 @@ void Ssimu2Handler::run(const uint8_t* const p1[3], const uint8_t* const p2[3], 
     }
     launch_all();
     VSHIP_HIP_CHECK(hipStreamSynchronize(stream_));
+    *h_score_ = host_score(layout_.maps, h_partials_, h_features_);
     have_run_ = true;
 }
 
@@ -601,8 +647,7 @@ double Ssimu2Handler::compute_device(const uint8_t* const src1[3], const uint8_t
 
 void Ssimu2Handler::last_features(double out[108])
 {
-    VSHIP_HIP_CHECK(hipSetDevice(device_));
-    VSHIP_HIP_CHECK(hipMemcpy(out, d_features_, 108 * sizeof(double), hipMemcpyDeviceToHost));
+    std::memcpy(out, h_features_, 108 * sizeof(double));
 }
 
 void Ssimu2Handler::debug_plane(int image, int plane, int scale, float* out, int64_t* w, int64_t* h)
@@ -623,22 +668,24 @@ void Ssimu2Handler::profile(int iters, double* ms_front, double* ms_blur, double
 {
     if (!have_run_) VSHIP_THROW(Vship_BadPointer, "profile needs one compute call first");
     VSHIP_HIP_CHECK(hipSetDevice(device_));
-    hipEvent_t ev[4];
+    hipEvent_t ev[3];
     for (auto& e : ev) VSHIP_HIP_CHECK(hipEventCreate(&e));
     double acc[3] = {0, 0, 0};
+    double feat[108];
     for (int it = 0; it < iters; it++) {
         VSHIP_HIP_CHECK(hipEventRecord(ev[0], stream_));
         VSHIP_HIP_CHECK(launch_frontend_single<float>(*h_front_jobs_, layout_.front_tiles_per_image, d_colors_,
                                                       d_geom_, call_spec(), stream_));
         VSHIP_HIP_CHECK(hipEventRecord(ev[1], stream_));
         VSHIP_HIP_CHECK(launch_blur<float>(layout_.blur_config, d_maps_, d_counts_, blur_grid_for(layout_.blur_tiles),
-                                           taps_, d_partials_, stream_));
+                                           taps_, h_partials_, stream_));
         VSHIP_HIP_CHECK(hipEventRecord(ev[2], stream_));
-        VSHIP_HIP_CHECK(launch_score(d_score_jobs_, d_counts_, 1, d_maps_, d_partials_, d_features_, d_score_,
-                                     stream_));
-        VSHIP_HIP_CHECK(hipEventRecord(ev[3], stream_));
-        VSHIP_HIP_CHECK(hipEventSynchronize(ev[3]));
-        for (int k = 0; k < 3; k++) {
+        VSHIP_HIP_CHECK(hipEventSynchronize(ev[2]));
+        // the score's FP64 completion on the host, as in run()
+        const auto t0 = std::chrono::steady_clock::now();
+        (void)host_score(layout_.maps, h_partials_, feat);
+        acc[2] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        for (int k = 0; k < 2; k++) {
             float ms = 0.0f;
             VSHIP_HIP_CHECK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
             acc[k] += ms;

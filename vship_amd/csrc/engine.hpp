@@ -4,8 +4,10 @@
 // pyramid geometry, the active (plane, scale) maps and the launch tables.
 // Ssimu2Handler owns one HIP stream and every device buffer one pair needs,
 // allocated once in the constructor (the reference allocates and frees
-// ~100 MB per call, src/ssimu2/main.hpp:126-152); compute() issues three
-// kernels (front-end, blur/maps, FP64 score) and one 8-byte read-back.
+// ~100 MB per call, src/ssimu2/main.hpp:126-152); compute() issues two
+// kernels (front-end, blur/maps) whose item partial sums land in pinned host
+// memory, and forms the score from them in FP64 on the calling thread, as the
+// reference completes its score on the host (src/ssimu2/score.hpp:380-460).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -73,8 +75,9 @@ public:
     void last_features(double out[108]);
     // Dense copy of one XYB plane of the last compute (image 0 or 1).
     void debug_plane(int image, int plane, int scale, float* out, int64_t* w, int64_t* h);
-    // Average device time (ms) of each kernel over `iters` replays of the last
-    // compute, measured with HIP events on the handler's stream.
+    // Average time (ms) of each step over `iters` replays of the last compute:
+    // the two kernels with HIP events on the handler's stream, the host's FP64
+    // completion of the score with a steady clock.
     void profile(int iters, double* ms_front, double* ms_blur, double* ms_score);
     // Algorithmic work of one pair (bytes and blur-stage FLOPs).
     void work(double* bytes, double* flops) const;
@@ -100,13 +103,13 @@ private:
     FrontColor* d_colors_ = nullptr;
     PairGeom* d_geom_ = nullptr;
     MapJob* d_maps_ = nullptr;
-    ScoreJob* d_score_jobs_ = nullptr;
     int* d_counts_ = nullptr;                  // {front tiles, blur tiles, pairs}
-    float* d_partials_ = nullptr;
-    double* d_features_ = nullptr;
-    double* d_score_ = nullptr;
     FrontJob* h_front_jobs_ = nullptr;         // pinned, one pair
     double* h_score_ = nullptr;                // pinned
+    // the blur items' partial sums of each call (pinned host memory the blur
+    // kernel writes) and the 108 features formed from them on the host
+    float* h_partials_ = nullptr;
+    double h_features_[108] = {};
     // crop/resize path, per image: linear RGB at the input size, after the
     // horizontal resize, after the vertical resize (3 planes each)
     float* lin_[2] = {nullptr, nullptr};

@@ -72,6 +72,9 @@ void score_kernel(const ScoreJob* __restrict__ pairs, const int* __restrict__ co
             if (lane + 64 < 108) s = fma(d_weights[lane + 64], feat[lane + 64], s);
             s = wave_sum_d(s);
             if (lane == 0) {
+                // no contraction: the single-pair handler forms the same
+                // polynomial on the host with these operations
+#pragma clang fp contract(off)
                 s *= 0.9562382616834844;
                 s = (6.248496625763138e-5 * s * s) * s + 2.326765642916932 * s - 0.020884521182843837 * s * s;
                 scores[pi] = (s > 0.0) ? pow(s, 0.6276336467831387) * -10.0 + 100.0 : 100.0;
