@@ -28,7 +28,7 @@ from tests import refvship
 pytestmark = pytest.mark.gpu
 
 FRAMES = 16
-CALLS = 96
+CALLS = 480  # long enough (tens of ms and more) that thread start-up and jitter do not dominate
 
 
 def _say(*a):
