@@ -248,8 +248,8 @@ def cpu_baseline(frames, threads: int, pairs: int):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--handlers", type=int, default=8)  # FFVship's default -g (GPU streams)
     ap.add_argument("--cpu-pairs", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=8)
