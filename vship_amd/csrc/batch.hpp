@@ -111,7 +111,6 @@ private:
     PairGeom* d_geoms_ = nullptr;
     MapJob* d_maps_ = nullptr;
     ScoreJob* d_sjobs_ = nullptr;
-    size_t tables_bytes_ = 0;
     int cap_fjobs_ = 0, cap_maps_ = 0;
 
     hipGraph_t graph_ = nullptr;
