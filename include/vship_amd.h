@@ -19,7 +19,8 @@ extern "C" {
 
 /* Vship_ComputeSSIMU2 on planes already resident in the memory of the
  * handler's GPU (device pointers, line sizes in bytes). No PCIe transfer
- * besides the 8-byte score. Blocks until *score is written. */
+ * besides the blur's 32 bytes per work item of partial sums, from which the
+ * calling thread forms the score in FP64. Blocks until *score is written. */
 EXPORTPREPROCESS Vship_Exception VshipAMD_ComputeSSIMU2Device(Vship_SSIMU2Handler handler, double* score,
                                                               const uint8_t* dsrcp1[3], const uint8_t* dsrcp2[3],
                                                               const int64_t lineSize[3], const int64_t lineSize2[3]);
