@@ -260,3 +260,98 @@ def test_mixed_resolution_batch_with_4k_10bit_vs_oracle():
         _say(f"mixed {w}x{h} {bits}-bit: fp16 {got16[i]:.5f} fp32 {got32[i]:.5f} oracle {want[i]:.5f}")
         assert abs(got16[i] - want[i]) <= 0.01
         assert abs(got32[i] - want[i]) <= 0.005
+
+
+class SharedSources:
+    """Reference images with several distorted candidates each (the same seed
+    draws the same reference at every noise level), for the screening layout
+    where many pairs repeat one source."""
+
+    NOISES = (0.01, 0.025, 0.05)
+
+    def __init__(self, sizes, kinds, seed):
+        self.cs, self.ref, self.dists, self.strides, self.bufs = [], [], [], [], []
+        for i, ((w, h), kind) in enumerate(zip(sizes, kinds)):
+            cs = S.colorspace(w, h, **cases.COLOR_CASES[kind][1])
+            ds = []
+            for noise in self.NOISES:
+                ref, dist, st = cases.perturbed_pair(cs, seed=seed + i, pad=3, noise=noise)
+                ds.append(dist)
+            ds.append(ref)  # identical candidate
+            self.cs.append(cs)
+            self.ref.append(ref)
+            self.dists.append(ds)
+            self.strides.append(st)
+
+    def upload(self, planes):
+        trip = []
+        for p in planes:
+            b = S.DeviceBuffer(p.nbytes)
+            b.upload(p)
+            self.bufs.append(b)
+            trip.append(b.address)
+        return trip
+
+    def order(self):
+        """(source, candidate) pairs, sources interleaved so that pairs which
+        repeat a source come before later first uses of other sources."""
+        out = []
+        for c in range(len(self.NOISES) + 1):
+            for s in range(len(self.cs)):
+                if c < len(self.NOISES) + 1 - (s % 2):
+                    out.append((s, c))
+        return out
+
+    def args(self, shared):
+        order = self.order()
+        ref_ptrs = [self.upload(r) for r in self.ref] if shared else None
+        cs = [self.cs[s] for s, _ in order]
+        p1 = [ref_ptrs[s] if shared else self.upload(self.ref[s]) for s, _ in order]
+        p2 = [self.upload(self.dists[s][c]) for s, c in order]
+        st = [self.strides[s] for s, _ in order]
+        return cs, cs, p1, p2, st, st
+
+    def free(self):
+        for b in self.bufs:
+            b.free()
+
+
+@pytest.mark.parametrize("kinds", [(0, 0, 0), (0, 1, 17)], ids=["layout", "generic"])
+@pytest.mark.parametrize("storage", [S.STORAGE_FP32, S.STORAGE_FP16])
+def test_shared_reference_pyramid(kinds, storage):
+    """Pairs that repeat a reference image (same descriptor, device planes and
+    line sizes) share its stored pyramid: their front-end work computes the
+    distorted image only, after the pair that computes the reference. Scores
+    and features equal those of the same batch with every reference uploaded
+    separately, bit for bit in both storages (the FP16 distorted planes are
+    formed from the reference's FP32 values either way); identical candidates
+    score 100; FP16 storage stays within 0.01 of the FP64 oracle."""
+    src = SharedSources([(160, 90), (97, 61), (64, 64)], kinds, seed=70)
+    try:
+        order = src.order()
+        n = len(order)
+        with S.SSIMU2Batch(n, storage) as b:
+            shared = b.compute(*src.args(shared=True))
+            f_shared = b.features(n)
+            separate = b.compute(*src.args(shared=False))
+            f_separate = b.features(n)
+            again = b.compute(*src.args(shared=True))
+        for i, (s, c) in enumerate(order):
+            _say(f"shared ref {src.cs[s].width}x{src.cs[s].height} kind {kinds[s]} cand {c}: "
+                 f"{shared[i]:.6f} separate {separate[i]:.6f}")
+        np.testing.assert_array_equal(shared, separate)
+        np.testing.assert_array_equal(f_shared, f_separate)
+        np.testing.assert_array_equal(again, shared)
+        for i, (s, c) in enumerate(order):
+            if c == len(src.NOISES):
+                assert shared[i] == 100.0, (s, shared[i])
+        if storage == S.STORAGE_FP16:
+            for i, (s, c) in enumerate(order):
+                if c != 1 or src.cs[s].width * src.cs[s].height < 64 * 64:
+                    continue
+                want, _ = oracle.ssimu2(src.cs[s], src.ref[s], src.strides[s], src.cs[s], src.dists[s][c],
+                                        src.strides[s])
+                _say(f"shared ref fp16 vs oracle {src.cs[s].width}x{src.cs[s].height}: {shared[i]:.6f} {want:.6f}")
+                assert abs(shared[i] - want) <= 0.01
+    finally:
+        src.free()

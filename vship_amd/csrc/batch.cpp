@@ -98,10 +98,17 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
     maps_.clear();
     sjobs_.clear();
     xyb_off_.clear();
+    exact_off_.assign(in.npairs, -1);
     const size_t elem = fp16_ ? 2 : 4;
     int64_t elems = 0;
-    int ftiles = 0, btiles = 0;
+    int btiles = 0;
     double in_bytes = 0.0, active = 0.0, flops = 0.0, blocks = 0.0;
+    // pairs whose reference image (descriptor, planes, line sizes) repeats an
+    // earlier pair's share its stored pyramid: their front-end job computes
+    // image 1 only, after the first pair's job (phase 1). In the FP16 layout
+    // the first pair's job also keeps image 0's FP32 values for them.
+    std::map<std::string, int> first_of;
+    std::vector<FrontJob> own_jobs, shared_jobs;
     for (int i = 0; i < in.npairs; i++) {
         const ImageSpec s1 = cached_spec(in.src[i]);
         const ImageSpec s2 = cached_spec(in.dis[i]);
@@ -119,15 +126,36 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
                     VSHIP_THROW(Vship_BadPointer, "lineSize smaller than one row of samples");
                 in_bytes += (double)specs[k]->plane_w[p] * specs[k]->plane_h[p] * specs[k]->bytes;
             }
+        std::string key(reinterpret_cast<const char*>(&in.src[i]), sizeof(Vship_Colorspace_t));
+        key.append(reinterpret_cast<const char*>(pl[0]), 3 * sizeof(pl[0][0]));
+        key.append(reinterpret_cast<const char*>(ls[0]), 3 * sizeof(ls[0][0]));
+        const auto found = first_of.find(key);
+        const int ref = found == first_of.end() ? -1 : found->second;
+        if (ref < 0) first_of.emplace(std::move(key), i);
         const PairLayout& L = cached_layout((int)s1.width, (int)s1.height, tuning_.blur_config);
         geoms_.push_back(L.geom);
         FrontJob j;
         std::memset(&j, 0, sizeof(j));
+        j.shared = ref >= 0 ? 1 : 0;
+        if (j.shared) {  // the raw reference planes are not read again
+            for (int p = 0; p < 3; p++) in_bytes -= (double)s1.plane_w[p] * s1.plane_h[p] * s1.bytes;
+            if (fp16_ && exact_off_[ref] < 0) {
+                // FP32 elements take two FP16 slots of the arena
+                elems = (int64_t)align_up((size_t)elems, 256 / elem);
+                exact_off_[ref] = elems;
+                elems += 2 * L.geom.image_elems;
+            }
+            exact_off_[i] = exact_off_[ref];
+        }
         for (int k = 0; k < 2; k++) {
-            // keep every image buffer 256-byte aligned
-            elems = (int64_t)align_up((size_t)elems, 256 / elem);
-            xyb_off_.push_back(elems);
-            elems += L.geom.image_elems;
+            if (k == 0 && j.shared) {
+                xyb_off_.push_back(xyb_off_[2 * ref]);
+            } else {
+                // keep every image buffer 256-byte aligned
+                elems = (int64_t)align_up((size_t)elems, 256 / elem);
+                xyb_off_.push_back(elems);
+                elems += L.geom.image_elems;
+            }
             for (int p = 0; p < 3; p++) {
                 j.plane[k][p] = pl[k][p];
                 j.stride[k][p] = ls[k][p];
@@ -138,10 +166,9 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
         j.width = (int32_t)s1.width;
         j.height = (int32_t)s1.height;
         j.tiles_x = L.front_tiles_x;
-        j.tile_start = ftiles;
+        j.tile_start = L.front_tiles_per_image;  // tile count until the jobs are ordered
         j.pair = i;
-        ftiles += L.front_tiles_per_image;
-        fjobs_.push_back(j);
+        (j.shared ? shared_jobs : own_jobs).push_back(j);
         ScoreJob sj = {(int32_t)maps_.size(), (int32_t)L.maps.size(), {0, 0}};
         sjobs_.push_back(sj);
         for (const MapJob& m0 : L.maps) {
@@ -161,6 +188,23 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
             blocks += std::ceil(m.w / 16.0) * std::ceil(m.h / 16.0);
         }
         btiles += L.blur_tiles;
+    }
+    // front-end jobs: the pairs that compute both images first (phase 0),
+    // then those whose image 0 a phase-0 job stores (phase 1)
+    int ftiles = 0;
+    for (auto* list : {&own_jobs, &shared_jobs}) {
+        if (list == &shared_jobs) front_shared_ = ftiles;
+        for (FrontJob j : *list) {
+            const int n = j.tile_start;
+            j.tile_start = ftiles;
+            ftiles += n;
+            fjobs_.push_back(j);
+        }
+    }
+    // the phase-1 launch is captured only when a job uses it
+    if (shared_jobs.empty() == front_phase1_) {
+        front_phase1_ = !shared_jobs.empty();
+        graph_valid_ = false;
     }
     // one front-end instantiation for the whole table: the layout every pair
     // shares, else the generic one (a change re-captures the graph)
@@ -206,8 +250,11 @@ void Ssimu2Batch::ensure_capacity()
         graph_valid_ = false;  // the kernels take the partials pointer as an argument
     }
     // final pointers into the arena, then one packed upload
-    for (size_t i = 0; i < fjobs_.size(); i++)
-        for (int k = 0; k < 2; k++) fjobs_[i].out[k] = static_cast<uint8_t*>(xyb_) + xyb_off_[2 * i + k] * elem;
+    for (FrontJob& j : fjobs_) {
+        for (int k = 0; k < 2; k++) j.out[k] = static_cast<uint8_t*>(xyb_) + xyb_off_[2 * j.pair + k] * elem;
+        const int64_t e = exact_off_[j.pair];
+        j.exact0 = e < 0 ? nullptr : reinterpret_cast<float*>(static_cast<uint8_t*>(xyb_) + e * elem);
+    }
     std::vector<MapJob> dev_maps = maps_;
     for (auto& m : dev_maps) {
         m.a = static_cast<const uint8_t*>(xyb_) + reinterpret_cast<int64_t>(m.a) * elem;
@@ -219,6 +266,7 @@ void Ssimu2Batch::ensure_capacity()
     counts[kCountBlurTiles] = blur_tiles_;
     counts[kCountBlurJobs] = (int)dev_maps.size();
     counts[kCountPairs] = npairs_;
+    counts[kCountFrontShared] = front_shared_;
     auto put = [&](const void* dev, const void* src, size_t bytes) {
         std::memcpy(h_tables_ + (static_cast<const uint8_t*>(dev) - tables_), src, bytes);
     };
@@ -238,10 +286,14 @@ void Ssimu2Batch::launch_kernels(hipStream_t s)
     const int bgrid = (std::max(1, tuning_.blur_waves) * cus_ + 7) / 8 * 8;
     const size_t lds = frontend_max_lds_bytes();
     if (fp16_) {
-        VSHIP_HIP_CHECK(launch_frontend<__half>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, front_spec_, s));
+        for (int phase = 0; phase < (front_phase1_ ? 2 : 1); phase++)
+            VSHIP_HIP_CHECK(
+                launch_frontend<__half>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, front_spec_, phase, s));
         VSHIP_HIP_CHECK(launch_blur<__half>(tuning_.blur_config, d_maps_, d_counts_, bgrid, taps_, partials_, s));
     } else {
-        VSHIP_HIP_CHECK(launch_frontend<float>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, front_spec_, s));
+        for (int phase = 0; phase < (front_phase1_ ? 2 : 1); phase++)
+            VSHIP_HIP_CHECK(
+                launch_frontend<float>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, front_spec_, phase, s));
         VSHIP_HIP_CHECK(launch_blur<float>(tuning_.blur_config, d_maps_, d_counts_, bgrid, taps_, partials_, s));
     }
     VSHIP_HIP_CHECK(launch_score(d_sjobs_, d_counts_, kScoreGrid, d_maps_, partials_, d_features_, d_scores_, s));
@@ -351,12 +403,14 @@ void Ssimu2Batch::profile(int iters, double* ms_front, double* ms_blur, double* 
     double acc[3] = {0, 0, 0};
     for (int it = 0; it < iters; it++) {
         VSHIP_HIP_CHECK(hipEventRecord(ev[0], stream_));
-        if (fp16_)
-            VSHIP_HIP_CHECK(
-                launch_frontend<__half>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, front_spec_, stream_));
-        else
-            VSHIP_HIP_CHECK(
-                launch_frontend<float>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, front_spec_, stream_));
+        for (int phase = 0; phase < (front_phase1_ ? 2 : 1); phase++) {
+            if (fp16_)
+                VSHIP_HIP_CHECK(launch_frontend<__half>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds,
+                                                        front_spec_, phase, stream_));
+            else
+                VSHIP_HIP_CHECK(launch_frontend<float>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds,
+                                                       front_spec_, phase, stream_));
+        }
         VSHIP_HIP_CHECK(hipEventRecord(ev[1], stream_));
         if (fp16_) VSHIP_HIP_CHECK(launch_blur<__half>(tuning_.blur_config, d_maps_, d_counts_, bgrid, taps_, partials_, stream_));
         else VSHIP_HIP_CHECK(launch_blur<float>(tuning_.blur_config, d_maps_, d_counts_, bgrid, taps_, partials_, stream_));

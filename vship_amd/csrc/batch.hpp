@@ -82,6 +82,9 @@ private:
     std::vector<MapJob> maps_;
     std::vector<ScoreJob> sjobs_;
     std::vector<int64_t> xyb_off_;   // per image (2 per pair): element offset in the arena
+    // per pair, FP16 layout: arena offset (FP16 elements) of the FP32 values of
+    // the shared image 0 its front-end job writes or reads, -1 if none
+    std::vector<int64_t> exact_off_;
     // descriptors and layouts of earlier calls (a batch usually repeats a few
     // colourspaces and sizes): keyed by the descriptor's bytes and by
     // (width, height, blur config)
@@ -117,6 +120,8 @@ private:
     hipGraphExec_t exec_ = nullptr;
     bool graph_valid_ = false;
     int front_spec_ = 0;  // FrontSpec of the front-end instantiation in use
+    int front_shared_ = 0;  // first front-end tile of the jobs with a shared image 0
+    bool front_phase1_ = false;  // the batch has such jobs: the front-end runs twice
 };
 
 }  // namespace vship

@@ -665,25 +665,25 @@ VSHIP_INLINE void to_linear(const FrontJob& job, int k, const FrontColor& fc, in
             int kx[kBW], ky[kBH];
             CrWeights wx[kBW], wy[kBH];
             bool cpx[kBW], cpy[kBH];
-    #pragma unroll
+#pragma unroll
             for (int i = 0; i < kBW; i++) {
                 float tt = 0.0f;
                 if (fc.subw) seg_of(x0 + i, fc.subw, fc.h_cosited != 0, kx[i], tt, cpx[i]);
                 else { kx[i] = x0 + i; cpx[i] = true; }
                 wx[i] = cr_weights(tt);
             }
-    #pragma unroll
+#pragma unroll
             for (int j = 0; j < kBH; j++) {
                 float tt = 0.0f;
                 if (fc.subh) seg_of(y0 + j, fc.subh, fc.v_cosited != 0, ky[j], tt, cpy[j]);
                 else { ky[j] = y0 + j; cpy[j] = true; }
                 wy[j] = cr_weights(tt);
             }
-    #pragma unroll
+#pragma unroll
             for (int p = 0; p < 2; p++) {
-    #pragma unroll
+#pragma unroll
                 for (int j = 0; j < kBH; j++) {
-    #pragma unroll
+#pragma unroll
                     for (int i = 0; i < kBW; i++) {
                         float val;
                         if (fc.subh == 0) {
@@ -960,6 +960,20 @@ VSHIP_INLINE void gstore2<__half>(void* base, uint32_t off, float v0, float v1)
         f16x2_t{(_Float16)v0, (_Float16)v1};
 }
 
+// Global-memory loads of one or two consecutive FP32 elements of a plane
+// layout (the FP32 values of an image computed by an earlier job).
+VSHIP_INLINE float gload1(const float* base, uint32_t off)
+{
+    return ((const __attribute__((address_space(1))) float*)base)[off];
+}
+
+VSHIP_INLINE void gload2(const float* base, uint32_t off, float& v0, float& v1)
+{
+    const f32x2_t v = *(const __attribute__((address_space(1))) f32x2_t*)((const __attribute__((address_space(1))) float*)base + off);
+    v0 = v[0];
+    v1 = v[1];
+}
+
 // Colour layouts with their own instantiation of the front-end: with the
 // sample type, subsampling, chroma siting, transfer and matrix steps known at
 // compile time the per-format switches disappear, the chroma pass drops its
@@ -992,6 +1006,23 @@ VSHIP_INLINE void frontend_tile(const FrontJob& job, const FrontColor* __restric
     float* work = lds + (kHalf ? kFrontStashFloats : 0);
     float* l1_area = work + kChromaLdsMax;  // [2 images][kL1Floats]
     const int t = threadIdx.x;
+    if constexpr (!kHalf) {
+        if (job.shared) {  // image 0 is stored already: image 1 only
+            TileXyb r;
+            float P[3][kBH][kBW];
+            if constexpr (SPEC == kFrontGeneric) {
+                compute_tile(job, 1, colors[job.color[1]], geo, tile_x, tile_y, work, l1_area + kL1Floats, r, P);
+            } else {
+                const FrontColor c1 = front_color<SPEC>(colors[job.color[1]]);
+                compute_tile<SPEC>(job, 1, c1, geo, tile_x, tile_y, work, l1_area + kL1Floats, r, P);
+            }
+            void* o1 = job.out[1];
+            store_tile(
+                job, geo, tile_x, tile_y, [&](int, uint32_t off, int f) { gstore1<StoreT>(o1, off, field(r, f)); },
+                [&](int, uint32_t off, int f) { gstore2<StoreT>(o1, off, field(r, f), field(r, f + 1)); });
+            return;
+        }
+    }
     if constexpr (!kHalf && SPEC == kFrontGeneric) {
 #pragma unroll 1
         for (int k = 0; k < 2; k++) {
@@ -1073,19 +1104,26 @@ VSHIP_INLINE int front_job_of(const FrontJob* __restrict__ jobs, int njobs, int 
 
 // The layout instantiations need fewer registers: 80 give 6 waves per SIMD,
 // i.e. three 8-wave workgroups per CU (LDS allows three in both layouts).
-template <typename StoreT, int SPEC>
+//
+// PHASE 0 covers tiles [0, first_shared), the jobs that compute both images;
+// PHASE 1 the rest, the jobs whose image 0 a phase-0 job stored (launched
+// after phase 0 on the same stream). Separate instantiations keep the
+// image-1-only code out of the register allocation of the common one.
+template <typename StoreT, int SPEC, int PHASE>
 __global__ __launch_bounds__(kFrontThreads, SPEC == kFrontGeneric ? 4 : 6)
 void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ counts,
                      const FrontColor* __restrict__ colors, const PairGeom* __restrict__ geoms)
 {
     extern __shared__ float lds[];
-    const int total = counts[kCountFrontTiles];
+    const int first_shared = counts[kCountFrontShared];
+    const int lo = PHASE == 0 ? 0 : first_shared;
+    const int hi = PHASE == 0 ? first_shared : counts[kCountFrontTiles];
     const int njobs = counts[kCountFrontJobs];
-    const int per = (total + kXcds - 1) / kXcds;
+    const int per = (hi - lo + kXcds - 1) / kXcds;
     const int xcd = blockIdx.x % kXcds;
-    const int end = min((xcd + 1) * per, total);
+    const int end = min(lo + (xcd + 1) * per, hi);
     const int step = gridDim.x / kXcds;
-    int item = xcd * per + blockIdx.x / kXcds;
+    int item = lo + xcd * per + blockIdx.x / kXcds;
     if constexpr (sizeof(StoreT) == 4) {
         for (; item < end; item += step) {
             const FrontJob& job = jobs[front_job_of(jobs, njobs, item)];
@@ -1104,49 +1142,78 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
             if constexpr (SPEC == kFrontGeneric) return colors[i];
             else return front_color<SPEC>(colors[i]);
         };
+        // raw samples of a tile's first image: image 1 when image 0 is shared
+        auto first_raw = [&](const FrontJob& j, int local) {
+            return raw_tile<kO32>(j, PHASE, color(j.color[PHASE]), local % j.tiles_x, local / j.tiles_x);
+        };
         RawTile raw = {};
         if (item < end) {
             const FrontJob& job = jobs[front_job_of(jobs, njobs, item)];
-            const int local = item - job.tile_start;
-            raw = raw_tile<kO32>(job, 0, color(job.color[0]), local % job.tiles_x, local / job.tiles_x);
+            raw = first_raw(job, item - job.tile_start);
         }
         for (; item < end; item += step) {
             const FrontJob& job = jobs[front_job_of(jobs, njobs, item)];
             const PairGeom& geo = geoms[job.pair];
             const int local = item - job.tile_start;
             const int tile_x = local % job.tiles_x, tile_y = local / job.tiles_x;
-            const FrontColor c0 = color(job.color[0]);
             const FrontColor c1 = color(job.color[1]);
             TileXyb a, b;
             float P[3][kBH][kBW];
-            unpack_raw(c0, raw, work, P);
-            raw = raw_tile<kO32>(job, 1, c1, tile_x, tile_y);  // in flight during image 0
-            compute_tile<SPEC>(job, 0, c0, geo, tile_x, tile_y, work, l1_area, a, P, true);
-#pragma unroll
-            for (int f = 0; f < kFields; f++) stash[f * kFrontThreads + t] = field(a, f);
-            unpack_raw(c1, raw, work, P);
             const int next = item + step;
-            if (next < end) {  // the next tile's image 0, in flight during image 1
-                const FrontJob& nj = jobs[front_job_of(jobs, njobs, next)];
-                const int nl = next - nj.tile_start;
-                raw = raw_tile<kO32>(nj, 0, color(nj.color[0]), nl % nj.tiles_x, nl / nj.tiles_x);
+            if constexpr (PHASE == 1) {
+                // image 0's FP32 values, stored by the job that computed it,
+                // in flight (registers) while image 1 is computed
+                const float* e0 = job.exact0;
+                float ev[kFields];
+                store_tile(
+                    job, geo, tile_x, tile_y, [&](int, uint32_t off, int f) { ev[f] = gload1(e0, off); },
+                    [&](int, uint32_t off, int f) { gload2(e0, off, ev[f], ev[f + 1]); });
+                unpack_raw(c1, raw, work, P);
+                if (next < end) {
+                    const FrontJob& nj = jobs[front_job_of(jobs, njobs, next)];
+                    raw = first_raw(nj, next - nj.tile_start);
+                }
+                compute_tile<SPEC>(job, 1, c1, geo, tile_x, tile_y, work, l1_area + kL1Floats, b, P, true);
+                void* o1 = job.out[1];
+                store_tile(
+                    job, geo, tile_x, tile_y,
+                    [&](int, uint32_t off, int f) { gstore1<StoreT>(o1, off, field(b, f) - ev[f]); },
+                    [&](int, uint32_t off, int f) {
+                        gstore2<StoreT>(o1, off, field(b, f) - ev[f], field(b, f + 1) - ev[f + 1]);
+                    });
+                __syncthreads();  // LDS is reused by the next item
+            } else {
+                const FrontColor c0 = color(job.color[0]);
+                unpack_raw(c0, raw, work, P);
+                raw = raw_tile<kO32>(job, 1, c1, tile_x, tile_y);  // in flight during image 0
+                compute_tile<SPEC>(job, 0, c0, geo, tile_x, tile_y, work, l1_area, a, P, true);
+#pragma unroll
+                for (int f = 0; f < kFields; f++) stash[f * kFrontThreads + t] = field(a, f);
+                unpack_raw(c1, raw, work, P);
+                if (next < end) {  // the next tile's first image, in flight during image 1
+                    const FrontJob& nj = jobs[front_job_of(jobs, njobs, next)];
+                    raw = first_raw(nj, next - nj.tile_start);
+                }
+                compute_tile<SPEC>(job, 1, c1, geo, tile_x, tile_y, work, l1_area + kL1Floats, b, P, true);
+                void* o0 = job.out[0];
+                void* o1 = job.out[1];
+                float* e0 = job.exact0;  // image 0 shared with later jobs: its FP32 values too
+                store_tile(
+                    job, geo, tile_x, tile_y,
+                    [&](int plane, uint32_t off, int f) {
+                        const float va = stash[f * kFrontThreads + t];
+                        gstore1<StoreT>(o0, off, va - kHalfOffset[plane]);
+                        gstore1<StoreT>(o1, off, field(b, f) - va);
+                        if (e0) gstore1<float>(e0, off, va);
+                    },
+                    [&](int plane, uint32_t off, int f) {
+                        const float va0 = stash[f * kFrontThreads + t], va1 = stash[(f + 1) * kFrontThreads + t];
+                        gstore2<StoreT>(o0, off, va0 - kHalfOffset[plane], va1 - kHalfOffset[plane]);
+                        gstore2<StoreT>(o1, off, field(b, f) - va0, field(b, f + 1) - va1);
+                        if (e0) gstore2<float>(e0, off, va0, va1);
+                    });
+                __syncthreads();  // LDS is reused by the next item
             }
-            compute_tile<SPEC>(job, 1, c1, geo, tile_x, tile_y, work, l1_area + kL1Floats, b, P, true);
-            void* o0 = job.out[0];
-            void* o1 = job.out[1];
-            store_tile(
-                job, geo, tile_x, tile_y,
-                [&](int plane, uint32_t off, int f) {
-                    const float va = stash[f * kFrontThreads + t];
-                    gstore1<StoreT>(o0, off, va - kHalfOffset[plane]);
-                    gstore1<StoreT>(o1, off, field(b, f) - va);
-                },
-                [&](int plane, uint32_t off, int f) {
-                    const float va0 = stash[f * kFrontThreads + t], va1 = stash[(f + 1) * kFrontThreads + t];
-                    gstore2<StoreT>(o0, off, va0 - kHalfOffset[plane], va1 - kHalfOffset[plane]);
-                    gstore2<StoreT>(o1, off, field(b, f) - va0, field(b, f + 1) - va1);
-                });
-            __syncthreads();  // LDS is reused by the next item
         }
     }
 }
@@ -1231,7 +1298,7 @@ size_t frontend_max_lds_bytes()
 
 template <typename StoreT>
 hipError_t launch_frontend(const FrontJob* jobs, const int* counts, int grid, const FrontColor* colors,
-                           const PairGeom* geoms, size_t lds_bytes, int spec, hipStream_t stream)
+                           const PairGeom* geoms, size_t lds_bytes, int spec, int phase, hipStream_t stream)
 {
     if (grid <= 0) return hipSuccess;
     grid = (grid + kXcds - 1) / kXcds * kXcds;  // the item mapping needs whole XCD rounds
@@ -1239,18 +1306,21 @@ hipError_t launch_frontend(const FrontJob* jobs, const int* counts, int grid, co
     (void)lds_bytes;
     const size_t bytes = sizeof(float) * ((size_t)kChromaLdsMax + 2 * kL1Floats +
                                           (sizeof(StoreT) == 2 ? (size_t)kFrontStashFloats : 0));
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(kFrontThreads), bytes, stream, jobs, counts, colors, geoms);
+    };
     switch (spec) {
     case kFrontU8Yuv420Bt709:
-        hipLaunchKernelGGL((frontend_kernel<StoreT, kFrontU8Yuv420Bt709>), dim3(grid), dim3(kFrontThreads), bytes,
-                           stream, jobs, counts, colors, geoms);
+        if (phase == 0) go(frontend_kernel<StoreT, kFrontU8Yuv420Bt709, 0>);
+        else go(frontend_kernel<StoreT, kFrontU8Yuv420Bt709, 1>);
         break;
     case kFrontU16Yuv420Bt709:
-        hipLaunchKernelGGL((frontend_kernel<StoreT, kFrontU16Yuv420Bt709>), dim3(grid), dim3(kFrontThreads), bytes,
-                           stream, jobs, counts, colors, geoms);
+        if (phase == 0) go(frontend_kernel<StoreT, kFrontU16Yuv420Bt709, 0>);
+        else go(frontend_kernel<StoreT, kFrontU16Yuv420Bt709, 1>);
         break;
     default:
-        hipLaunchKernelGGL((frontend_kernel<StoreT, kFrontGeneric>), dim3(grid), dim3(kFrontThreads), bytes, stream,
-                           jobs, counts, colors, geoms);
+        if (phase == 0) go(frontend_kernel<StoreT, kFrontGeneric, 0>);
+        else go(frontend_kernel<StoreT, kFrontGeneric, 1>);
         break;
     }
     return hipGetLastError();
@@ -1310,8 +1380,8 @@ template hipError_t launch_frontend_single<float>(const FrontJob&, int, const Fr
                                                   hipStream_t);
 
 template hipError_t launch_frontend<float>(const FrontJob*, const int*, int, const FrontColor*, const PairGeom*, size_t,
-                                           int, hipStream_t);
+                                           int, int, hipStream_t);
 template hipError_t launch_frontend<__half>(const FrontJob*, const int*, int, const FrontColor*, const PairGeom*, size_t,
-                                            int, hipStream_t);
+                                            int, int, hipStream_t);
 
 }  // namespace vship

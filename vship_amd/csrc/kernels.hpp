@@ -29,9 +29,12 @@ bool front_offsets_fit(const FrontJob& job, const FrontColor& c0, const FrontCol
 // the kernels loop over items (persistent mode).
 // spec: the FrontSpec every image of the job table shares (kFrontGeneric
 // when they differ).
+// phase 0 runs tiles [0, counts[kCountFrontShared]), phase 1 the rest: the
+// jobs whose image 0 (FrontJob::shared) a phase-0 job stores, which must have
+// completed (stream order).
 template <typename StoreT>
 hipError_t launch_frontend(const FrontJob* jobs, const int* counts, int grid, const FrontColor* colors,
-                           const PairGeom* geoms, size_t lds_bytes, int spec, hipStream_t stream);
+                           const PairGeom* geoms, size_t lds_bytes, int spec, int phase, hipStream_t stream);
 
 
 // One pair's tiles (tiles = the job's 64x32 tile count) with the job passed

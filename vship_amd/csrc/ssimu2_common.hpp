@@ -56,6 +56,7 @@ enum CountSlot : int32_t {
     kCountBlurTiles = 2,
     kCountBlurJobs = 3,
     kCountPairs = 4,
+    kCountFrontShared = 5,  // first front-end tile of the jobs with a shared image 0
     kCountSlots = 8,
 };
 
@@ -115,7 +116,14 @@ struct FrontJob {
     int32_t tiles_x;             // 64-wide tiles along x
     int32_t tile_start;          // first global tile index of this job
     int32_t pair;                // index into the PairGeom table
-    int32_t pad;
+    int32_t shared;              // 1: image 0 was computed by an earlier job
+                                 // (out[0] points at its stored planes)
+    // FP16 layout, image 0 shared with later jobs: the FP32 values of image 0
+    // (the stored plane layout, float elements), written by the job that
+    // computes image 0 and read by the jobs that share it, so that their
+    // distorted planes fp16(b - a) are the same as without sharing; null
+    // otherwise
+    float* exact0;
 };
 
 // The stored XYB planes per scale (plane X, Y, B), fixed by the weight table:
