@@ -463,15 +463,24 @@ class Screening:
                 row.append(trip)
             self.cand_ptr.append(row)
         self.pixels = width * height
+        self._prepared = {}  # call arguments of the full candidate list
 
     def _run(self, batch, items):
         """Exact scores and features of (source, candidate) items in one
-        batched call (the source planes are shared, not copied)."""
-        css = [self.cs] * len(items)
-        p1 = [self.ref_ptr[i] for i, _ in items]
-        p2 = [self.cand_ptr[i][k] for i, k in items]
-        st = [self.strides] * len(items)
-        scores = batch.compute(css, css, p1, p2, st, st)
+        batched call. The source planes are passed once per source (the same
+        device pointers), so the batch computes each source's pyramid once;
+        the call arguments of the full candidate list are built once."""
+        key = tuple(items)
+        args = self._prepared.get(key)
+        if args is None:
+            css = [self.cs] * len(items)
+            p1 = [self.ref_ptr[i] for i, _ in items]
+            p2 = [self.cand_ptr[i][k] for i, k in items]
+            st = [self.strides] * len(items)
+            args = batch.prepare(css, css, p1, p2, st, st)
+            if len(items) == self.nsrc * self.ncand:
+                self._prepared[key] = args
+        scores = batch.compute_prepared(args, np.zeros(len(items), dtype=np.float64))
         return scores, batch.features(len(items))
 
     def exact_only(self, batch, target):
