@@ -102,7 +102,7 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
     const size_t elem = fp16_ ? 2 : 4;
     int64_t elems = 0;
     int btiles = 0;
-    double in_bytes = 0.0, active = 0.0, flops = 0.0, blocks = 0.0;
+    double in_bytes = 0.0, active = 0.0, written = 0.0, flops = 0.0, blocks = 0.0;
     // pairs whose reference image (descriptor, planes, line sizes) repeats an
     // earlier pair's share its stored pyramid: their front-end job computes
     // image 1 only, after the first pair's job (phase 1). In the FP16 layout
@@ -181,6 +181,7 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
             maps_.push_back(m);
             const double px = (double)m.w * m.h;
             active += px;
+            written += j.shared ? px : 2.0 * px;  // a shared image 0 is stored once
             const bool ssim = m.flags & kMapSsim;
             const bool edge = m.flags & (kMapArtifact | kMapDetail);
             const int n_edge = ((m.flags & kMapArtifact) ? 1 : 0) + ((m.flags & kMapDetail) ? 1 : 0);
@@ -223,7 +224,9 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
     front_tiles_ = ftiles;
     blur_tiles_ = btiles;
     npairs_ = in.npairs;
-    work_bytes_ = in_bytes + 2.0 * 2.0 * (double)elem * active + 48.0 * blocks;
+    // raw inputs read, stored planes written by the front-end and read twice
+    // (both images) by the blur, 48 B of partials per 16x16 block
+    work_bytes_ = in_bytes + (double)elem * (written + 2.0 * active) + 48.0 * blocks;
     work_flops_ = flops;
 }
 
