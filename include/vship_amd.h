@@ -83,7 +83,9 @@ EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2BatchFree(VshipAMD_SSIMU2Batch b
  * batch's GPU. Pair i uses src_cs[i] / dis_cs[i] (any sizes, the two of a
  * pair equal), planes dsrcp1[3i..3i+2] / dsrcp2[3i..3i+2] and line sizes
  * lineSize1[3i..] / lineSize2[3i..]. Blocks until scores[0..npairs) are
- * written. */
+ * written. Pairs whose reference (src_cs bytes, the three dsrcp1 pointers and
+ * line sizes) repeats an earlier pair's reuse that pair's stored reference
+ * pyramid (candidate screening); the results are the same bit for bit. */
 EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2BatchCompute(VshipAMD_SSIMU2Batch batch, int npairs,
                                                              const Vship_Colorspace_t* src_cs,
                                                              const Vship_Colorspace_t* dis_cs,
