@@ -11,6 +11,11 @@ python -m vship_amd.profile_summary valu KERNEL_DB SIMDS XCDS
     (a wave64 VALU instruction holds a 16-lane SIMD for 4 cycles) summed over
     the chip, GRBM_GUI_ACTIVE counts the dispatch's cycles once per XCD, so
     busy = 4 * SQ_ACTIVE_INST_VALU / (SIMDS * GRBM_GUI_ACTIVE / XCDS)
+python -m vship_amd.profile_summary busy RESULTS_DB WINDOW [NAME...]
+    GPU occupancy of a kernel trace, per window of WINDOW consecutive
+    dispatches of the kernels whose names contain one of NAME (all kernels
+    when none is given): the window's span, the fraction of it during which
+    at least one of them runs, and the mean number running at once
 """
 from __future__ import annotations
 
@@ -37,6 +42,27 @@ def pmc(counter: str, db: str) -> dict[str, float]:
     for name, _d, v in c.execute(q, (counter,)):
         acc.setdefault(_short(name), []).append(v)
     return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def busy(db: str, window: int, names: list[str]) -> list[tuple[int, float, float, float]]:
+    """(first dispatch, span ms, busy fraction, mean kernels in flight) per window."""
+    c = sqlite3.connect(db)
+    rows = [(s, e) for s, e, n in c.execute("select start, end, name from kernels order by start")
+            if not names or any(x in n for x in names)]
+    out = []
+    for k in range(0, len(rows) - window + 1, window):
+        iv = sorted(rows[k:k + window])
+        t0, t1 = iv[0][0], max(e for _, e in iv)
+        covered, cs, ce = 0, iv[0][0], iv[0][1]
+        for s, e in iv[1:]:
+            if s > ce:
+                covered += ce - cs
+                cs, ce = s, e
+            else:
+                ce = max(ce, e)
+        covered += ce - cs
+        out.append((k, (t1 - t0) / 1e6, covered / (t1 - t0), sum(e - s for s, e in iv) / (t1 - t0)))
+    return out
 
 
 def main(argv=None) -> int:
@@ -66,6 +92,12 @@ def main(argv=None) -> int:
                 continue
             cyc = grbm[k] / xcds
             print(f"| `{k}` | {ins[k]:.0f} | {act[k]:.0f} | {cyc:.0f} | {4.0 * act[k] / (simds * cyc):.3f} |")
+        return 0
+    if a and a[0] == "busy":
+        print("| first dispatch | span ms | GPU busy | kernels in flight (mean) |")
+        print("|---|---|---|---|")
+        for k, span, frac, conc in busy(a[1], int(a[2]), a[3:]):
+            print(f"| {k} | {span:.3f} | {frac:.3f} | {conc:.2f} |")
         return 0
     print(__doc__)
     return 2
