@@ -3,6 +3,7 @@
 python -m vship_amd.profile_workload plain --pairs 8     # plain FP32 path, 1080p YUV420
 python -m vship_amd.profile_workload batch --pairs 240   # batched FP16 path, 1080p YUV420
 python -m vship_amd.profile_workload batch --blur-config 5  # same, matrix-core blur
+python -m vship_amd.profile_workload batch --sources 16  # 240 pairs over 16 shared reference images
 python -m vship_amd.profile_workload copy --mib 256      # plain copy kernel (counter calibration)
 
 Each mode prints what it ran and the HIP-event time per kernel.
@@ -49,6 +50,9 @@ def main(argv=None):
     ap.add_argument("--blur-config", type=int, default=None, help="batch: blur tile config (5-8 = matrix cores)")
     ap.add_argument("--blur-waves", type=int, default=None, help="batch: blur workgroups per CU")
     ap.add_argument("--front-waves", type=int, default=None, help="batch: front-end workgroups per CU")
+    ap.add_argument("--sources", type=int, default=0,
+                    help="batch: pairs share the reference planes of this many sources, in contiguous groups "
+                         "(0: every pair has its own reference)")
     a = ap.parse_args(argv)
     if a.mode == "copy":
         nbytes = a.mib << 20
@@ -76,8 +80,9 @@ def main(argv=None):
                 b.set_tuning(t["blur_config"] if a.blur_config is None else a.blur_config,
                              t["blur_waves"] if a.blur_waves is None else a.blur_waves,
                              t["front_waves"] if a.front_waves is None else a.front_waves)
-            args = ([cs] * a.pairs, [cs] * a.pairs, [p[0] for p in ptrs], [p[1] for p in ptrs], [st] * a.pairs,
-                    [st] * a.pairs)
+            group = -(-a.pairs // a.sources) if a.sources > 0 else 1
+            refs = [ptrs[i // group * group][0] for i in range(a.pairs)]  # same device planes within a group
+            args = ([cs] * a.pairs, [cs] * a.pairs, refs, [p[1] for p in ptrs], [st] * a.pairs, [st] * a.pairs)
             for _ in range(a.reps):
                 scores = b.compute(*args)
             fr, bl, sc, gr = b.profile(a.reps)
