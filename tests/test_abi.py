@@ -118,3 +118,50 @@ def test_pinned_free_of_garbage_pointer_is_an_error_not_a_crash():
         pytest.skip("only meaningful without a GPU runtime")
     lib = S.load_library()
     assert lib.Vship_PinnedFree(ctypes.c_void_p(0x1234)) in (11,)
+
+
+def test_free_from_a_second_thread_while_computing():
+    # Without a GPU the init fails but hands out an id (as the reference does),
+    # so the handle table is exercised through the real entry points: one
+    # thread computes on ids that another frees and re-creates. Every call
+    # must return an error code (BadHandler, or the init's device error),
+    # never crash. On a GPU box the handlers are real and compute fails on
+    # the null planes (BadPointer) instead.
+    import threading
+    lib = S.load_library()
+    cs = S.colorspace(64, 64)
+    ids = []
+    for _ in range(4):
+        h = S.Handler()
+        lib.Vship_SSIMU2Init2(ctypes.byref(h), cs, cs, 0)
+        ids.append(h.id)
+    stop = threading.Event()
+    codes = set()
+
+    def compute():
+        score = ctypes.c_double()
+        planes = (ctypes.c_void_p * 3)()
+        ls = (ctypes.c_int64 * 3)(64, 32, 32)
+        k = 0
+        while not stop.is_set():
+            codes.add(lib.Vship_ComputeSSIMU2(S.Handler(ids[k % 4]), ctypes.byref(score), planes, planes, ls, ls))
+            k += 1
+
+    def churn():
+        for it in range(2000):
+            j = it % 4
+            lib.Vship_SSIMU2Free(S.Handler(ids[j]))
+            h = S.Handler()
+            lib.Vship_SSIMU2Init2(ctypes.byref(h), cs, cs, 0)
+            ids[j] = h.id
+
+    workers = [threading.Thread(target=compute) for _ in range(3)]
+    for w in workers:
+        w.start()
+    churn()
+    stop.set()
+    for w in workers:
+        w.join()
+    for i in ids:
+        lib.Vship_SSIMU2Free(S.Handler(i))
+    assert codes and codes <= {6, 7, 10, 11}, codes

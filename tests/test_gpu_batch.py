@@ -355,3 +355,25 @@ def test_shared_reference_pyramid(kinds, storage):
                 assert abs(shared[i] - want) <= 0.01
     finally:
         src.free()
+
+
+def test_failed_compute_leaves_no_stale_results(pairs):
+    """A compute that fails (here a null plane pointer, rejected on the host
+    before any launch) leaves no completed batch behind: features() and
+    profile() return BadPointer instead of reading the previous batch's
+    pairs, and the context keeps working."""
+    n = len(SIZES)
+    with S.SSIMU2Batch(16, S.STORAGE_FP16) as b:
+        want = b.compute(*pairs.args())
+        src, dis, p1, p2, l1, l2 = pairs.args([0, 1])
+        p2 = [p2[0], [0, 0, 0]]
+        with pytest.raises(S.VshipError) as e:
+            b.compute(src, dis, p1, p2, l1, l2)
+        assert e.value.code == 11
+        for call in (lambda: b.features(2), lambda: b.profile(1)):
+            with pytest.raises(S.VshipError) as e:
+                call()
+            assert e.value.code == 11
+        got = b.compute(*pairs.args())
+        assert np.array_equal(got, want)
+        assert b.features(n).shape == (n, 108)

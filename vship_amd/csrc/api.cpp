@@ -3,9 +3,11 @@
 // Behaviour follows the reference's src/VshipLib.cpp:70-563: every entry point
 // returns a Vship_Exception, records the detailed message of a failure
 // globally and per handler, and never lets a C++ exception escape. Handler ids
-// index a table guarded by a mutex; unlike the reference (VshipLib.cpp:242-256)
+// index tables (handle_table.hpp); unlike the reference (VshipLib.cpp:242-256)
 // a freed id is marked empty, so using it again returns Vship_BadHandler
-// instead of touching freed memory.
+// instead of touching freed memory, a call holds a reference to its handler so
+// that a concurrent Free cannot destroy it mid-call, and calls on one handler
+// are serialised.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,6 +23,7 @@
 #include "batch.hpp"
 #include "engine.hpp"
 #include "errors.hpp"
+#include "handle_table.hpp"
 #include "kernels.hpp"
 #include "mlp.hpp"
 
@@ -59,16 +62,20 @@ std::string Error::message() const
 
 namespace {
 
-struct HandlerSlot {
-    std::unique_ptr<Ssimu2Handler> impl;
-    std::string last_error;
-    bool live = false;
+// One object of a table plus the lock that serialises calls on it: a handler
+// is not reentrant (the reference leaves concurrent use of one handler
+// undefined, src/VshipLib.cpp:258); here the second call waits.
+template <typename T>
+struct Guarded {
+    explicit Guarded(std::unique_ptr<T> p) : impl(std::move(p)) {}
+    std::unique_ptr<T> impl;  // null when its init failed
+    std::mutex mu;
 };
 
-std::mutex g_lock;
-std::vector<HandlerSlot> g_handlers;
-std::vector<std::unique_ptr<Ssimu2Batch>> g_batches;
-std::vector<std::unique_ptr<Surrogate>> g_mlps;
+std::mutex g_lock;  // g_last_error, g_device
+HandleTable<Guarded<Ssimu2Handler>> g_handlers;
+HandleTable<Guarded<Ssimu2Batch>> g_batches;
+HandleTable<Guarded<Surrogate>> g_mlps;
 std::string g_last_error = Error(Vship_NoError, __FILE__, __LINE__).message();
 int g_device = 0;
 
@@ -97,20 +104,13 @@ int device_count()
 
 __global__ void probe_kernel(int* p) { p[0] = 4320984; }
 
-// Looks a live handler up; throws BadHandler otherwise.
-Ssimu2Handler* lookup(unsigned id, const char* where)
-{
-    std::lock_guard<std::mutex> g(g_lock);
-    if (id >= g_handlers.size() || !g_handlers[id].live)
-        throw Error(Vship_BadHandler, __FILE__, __LINE__, std::string("invalid handler id in ") + where);
-    return g_handlers[id].impl.get();
-}
-
 void record_handler_error(unsigned id, const std::string& msg)
 {
-    std::lock_guard<std::mutex> g(g_lock);
-    g_last_error = msg;
-    if (id < g_handlers.size() && g_handlers[id].live) g_handlers[id].last_error = msg;
+    {
+        std::lock_guard<std::mutex> g(g_lock);
+        g_last_error = msg;
+    }
+    (void)g_handlers.set_error(id, msg);
 }
 
 template <typename F>
@@ -149,19 +149,33 @@ Vship_Exception guarded_handler(unsigned id, F&& f)
     return Vship_NoError;
 }
 
-Ssimu2Batch* lookup_batch(unsigned id)
+// Runs f(object) holding a reference to a live table entry and its call
+// lock, so a concurrent Free cannot destroy the object under the call.
+template <typename T, typename F>
+void with_object(const HandleTable<Guarded<T>>& table, unsigned id, const char* where, F&& f)
 {
-    std::lock_guard<std::mutex> g(g_lock);
-    if (id >= g_batches.size() || !g_batches[id])
-        throw Error(Vship_BadHandler, __FILE__, __LINE__, "invalid batch id");
-    return g_batches[id].get();
+    std::shared_ptr<Guarded<T>> h;
+    if (!table.lookup(id, &h)) throw Error(Vship_BadHandler, __FILE__, __LINE__, std::string("invalid id in ") + where);
+    std::lock_guard<std::mutex> g(h->mu);
+    if (!h->impl) throw Error(Vship_BadHandler, __FILE__, __LINE__, std::string("initialisation had failed: ") + where);
+    f(*h->impl);
 }
 
-Surrogate* lookup_mlp(unsigned id)
+// The same for SSIMU2 handlers, recording a failure as the handler's detailed
+// error too (reference src/VshipLib.cpp:262-286).
+template <typename F>
+Vship_Exception handler_call(unsigned id, const char* where, F&& f)
 {
-    std::lock_guard<std::mutex> g(g_lock);
-    if (id >= g_mlps.size() || !g_mlps[id]) throw Error(Vship_BadHandler, __FILE__, __LINE__, "invalid surrogate id");
-    return g_mlps[id].get();
+    std::shared_ptr<Guarded<Ssimu2Handler>> h;
+    if (!g_handlers.lookup(id, &h)) {
+        set_last(Error(Vship_BadHandler, __FILE__, __LINE__, std::string("invalid handler id in ") + where).message());
+        return Vship_BadHandler;
+    }
+    return guarded_handler(id, [&] {
+        std::lock_guard<std::mutex> g(h->mu);
+        if (!h->impl) VSHIP_THROW(Vship_BadHandler, "handler initialisation had failed");
+        f(*h->impl);
+    });
 }
 
 BatchInput make_batch_input(int npairs, const Vship_Colorspace_t* src, const Vship_Colorspace_t* dis,
@@ -300,15 +314,14 @@ Vship_Exception Vship_SSIMU2Init2(Vship_SSIMU2Handler* handler, Vship_Colorspace
     });
     // like the reference, an id is handed out even when init fails, so that
     // Vship_SSIMU2GetDetailedLastError can report it; compute on it fails.
-    std::lock_guard<std::mutex> g(g_lock);
-    unsigned id = 0;
-    while (id < g_handlers.size() && g_handlers[id].live) id++;
-    if (id == g_handlers.size()) g_handlers.emplace_back();
-    HandlerSlot& slot = g_handlers[id];
-    slot.impl = std::move(impl);
-    slot.live = true;
-    slot.last_error = (err == Vship_NoError) ? Error(Vship_NoError, __FILE__, __LINE__).message() : g_last_error;
-    handler->id = id;
+    std::string msg;
+    if (err == Vship_NoError) {
+        msg = Error(Vship_NoError, __FILE__, __LINE__).message();
+    } else {
+        std::lock_guard<std::mutex> g(g_lock);
+        msg = g_last_error;
+    }
+    handler->id = g_handlers.insert(std::make_shared<Guarded<Ssimu2Handler>>(std::move(impl)), std::move(msg));
     return err;
 }
 
@@ -325,45 +338,33 @@ Vship_Exception Vship_SSIMU2Init(Vship_SSIMU2Handler* handler, Vship_Colorspace_
 
 Vship_Exception Vship_SSIMU2Free(Vship_SSIMU2Handler handler)
 {
-    std::unique_ptr<Ssimu2Handler> impl;
-    {
-        std::lock_guard<std::mutex> g(g_lock);
-        if (handler.id >= g_handlers.size() || !g_handlers[handler.id].live) {
-            g_last_error = Error(Vship_BadHandler, __FILE__, __LINE__,
-                                 "Handler in Vship_SSIMU2Free is not valid: never allocated or already freed").message();
-            return Vship_BadHandler;
-        }
-        impl = std::move(g_handlers[handler.id].impl);
-        g_handlers[handler.id].live = false;
-        g_handlers[handler.id].last_error.clear();
+    std::shared_ptr<Guarded<Ssimu2Handler>> h;
+    if (!g_handlers.remove(handler.id, &h)) {
+        set_last(Error(Vship_BadHandler, __FILE__, __LINE__,
+                       "Handler in Vship_SSIMU2Free is not valid: never allocated or already freed").message());
+        return Vship_BadHandler;
     }
-    return guarded([&] { impl.reset(); });
+    // a call still running on another thread holds its own reference: the
+    // handler is destroyed when that call returns
+    return guarded([&] { h.reset(); });
 }
 
 Vship_Exception Vship_ComputeSSIMU2(Vship_SSIMU2Handler handler, double* score, const uint8_t* srcp1[3],
                                     const uint8_t* srcp2[3], const int64_t lineSize[3], const int64_t lineSize2[3])
 {
-    Ssimu2Handler* h = nullptr;
-    Vship_Exception e = guarded([&] { h = lookup(handler.id, "Vship_ComputeSSIMU2"); });
-    if (e != Vship_NoError) return e;
-    return guarded_handler(handler.id, [&] {
-        if (!h) VSHIP_THROW(Vship_BadHandler, "handler initialisation had failed");
+    return handler_call(handler.id, "Vship_ComputeSSIMU2", [&](Ssimu2Handler& h) {
         if (!score || !srcp1 || !srcp2 || !lineSize || !lineSize2) VSHIP_THROW(Vship_BadPointer, "null argument");
-        *score = h->compute_host(srcp1, srcp2, lineSize, lineSize2);
+        *score = h.compute_host(srcp1, srcp2, lineSize, lineSize2);
     });
 }
 
 int Vship_SSIMU2GetDetailedLastError(Vship_SSIMU2Handler handler, char* out_message, int len)
 {
     std::string s;
-    {
-        std::lock_guard<std::mutex> g(g_lock);
-        if (handler.id >= g_handlers.size() || !g_handlers[handler.id].live) {
-            g_last_error = Error(Vship_BadHandler, __FILE__, __LINE__,
-                                 "Handler in Vship_SSIMU2GetDetailedLastError is not valid").message();
-            return 0;
-        }
-        s = g_handlers[handler.id].last_error;
+    if (!g_handlers.get_error(handler.id, &s)) {
+        set_last(Error(Vship_BadHandler, __FILE__, __LINE__, "Handler in Vship_SSIMU2GetDetailedLastError is not valid")
+                     .message());
+        return 0;
     }
     return copy_message(s, out_message, len);
 }
@@ -422,63 +423,43 @@ Vship_Exception VshipAMD_ComputeSSIMU2Device(Vship_SSIMU2Handler handler, double
                                              const uint8_t* dsrcp2[3], const int64_t lineSize[3],
                                              const int64_t lineSize2[3])
 {
-    Ssimu2Handler* h = nullptr;
-    Vship_Exception e = guarded([&] { h = lookup(handler.id, "VshipAMD_ComputeSSIMU2Device"); });
-    if (e != Vship_NoError) return e;
-    return guarded_handler(handler.id, [&] {
-        if (!h) VSHIP_THROW(Vship_BadHandler, "handler initialisation had failed");
+    return handler_call(handler.id, "VshipAMD_ComputeSSIMU2Device", [&](Ssimu2Handler& h) {
         if (!score || !dsrcp1 || !dsrcp2 || !lineSize || !lineSize2) VSHIP_THROW(Vship_BadPointer, "null argument");
-        *score = h->compute_device(dsrcp1, dsrcp2, lineSize, lineSize2);
+        *score = h.compute_device(dsrcp1, dsrcp2, lineSize, lineSize2);
     });
 }
 
 Vship_Exception VshipAMD_SSIMU2Features(Vship_SSIMU2Handler handler, double features[108])
 {
-    Ssimu2Handler* h = nullptr;
-    Vship_Exception e = guarded([&] { h = lookup(handler.id, "VshipAMD_SSIMU2Features"); });
-    if (e != Vship_NoError) return e;
-    return guarded_handler(handler.id, [&] {
-        if (!h) VSHIP_THROW(Vship_BadHandler, "handler initialisation had failed");
+    return handler_call(handler.id, "VshipAMD_SSIMU2Features", [&](Ssimu2Handler& h) {
         if (!features) VSHIP_THROW(Vship_BadPointer, "null argument");
-        h->last_features(features);
+        h.last_features(features);
     });
 }
 
 Vship_Exception VshipAMD_SSIMU2DebugPlane(Vship_SSIMU2Handler handler, int image, int plane, int scale, float* out,
                                           int64_t* width, int64_t* height)
 {
-    Ssimu2Handler* h = nullptr;
-    Vship_Exception e = guarded([&] { h = lookup(handler.id, "VshipAMD_SSIMU2DebugPlane"); });
-    if (e != Vship_NoError) return e;
-    return guarded_handler(handler.id, [&] {
-        if (!h) VSHIP_THROW(Vship_BadHandler, "handler initialisation had failed");
+    return handler_call(handler.id, "VshipAMD_SSIMU2DebugPlane", [&](Ssimu2Handler& h) {
         if (!width || !height) VSHIP_THROW(Vship_BadPointer, "null argument");
-        h->debug_plane(image, plane, scale, out, width, height);
+        h.debug_plane(image, plane, scale, out, width, height);
     });
 }
 
 Vship_Exception VshipAMD_SSIMU2Profile(Vship_SSIMU2Handler handler, int iters, double* ms_frontend, double* ms_blur,
                                        double* ms_score)
 {
-    Ssimu2Handler* h = nullptr;
-    Vship_Exception e = guarded([&] { h = lookup(handler.id, "VshipAMD_SSIMU2Profile"); });
-    if (e != Vship_NoError) return e;
-    return guarded_handler(handler.id, [&] {
-        if (!h) VSHIP_THROW(Vship_BadHandler, "handler initialisation had failed");
+    return handler_call(handler.id, "VshipAMD_SSIMU2Profile", [&](Ssimu2Handler& h) {
         if (iters <= 0 || !ms_frontend || !ms_blur || !ms_score) VSHIP_THROW(Vship_BadPointer, "bad argument");
-        h->profile(iters, ms_frontend, ms_blur, ms_score);
+        h.profile(iters, ms_frontend, ms_blur, ms_score);
     });
 }
 
 Vship_Exception VshipAMD_SSIMU2Work(Vship_SSIMU2Handler handler, double* bytes, double* flops)
 {
-    Ssimu2Handler* h = nullptr;
-    Vship_Exception e = guarded([&] { h = lookup(handler.id, "VshipAMD_SSIMU2Work"); });
-    if (e != Vship_NoError) return e;
-    return guarded_handler(handler.id, [&] {
-        if (!h) VSHIP_THROW(Vship_BadHandler, "handler initialisation had failed");
+    return handler_call(handler.id, "VshipAMD_SSIMU2Work", [&](Ssimu2Handler& h) {
         if (!bytes || !flops) VSHIP_THROW(Vship_BadPointer, "null argument");
-        h->work(bytes, flops);
+        h.work(bytes, flops);
     });
 }
 
@@ -491,25 +472,16 @@ Vship_Exception VshipAMD_SSIMU2BatchInit(VshipAMD_SSIMU2Batch* batch, int max_pa
         if (storage != VshipAMD_StorageFP32 && storage != VshipAMD_StorageFP16)
             VSHIP_THROW(Vship_BadPointer, "storage must be VshipAMD_StorageFP32 or VshipAMD_StorageFP16");
         std::unique_ptr<Ssimu2Batch> b(new Ssimu2Batch(max_pairs, storage == VshipAMD_StorageFP16, gpu_id));
-        std::lock_guard<std::mutex> g(g_lock);
-        unsigned id = 0;
-        while (id < g_batches.size() && g_batches[id]) id++;
-        if (id == g_batches.size()) g_batches.emplace_back();
-        g_batches[id] = std::move(b);
-        batch->id = id;
+        batch->id = g_batches.insert(std::make_shared<Guarded<Ssimu2Batch>>(std::move(b)));
     });
 }
 
 Vship_Exception VshipAMD_SSIMU2BatchFree(VshipAMD_SSIMU2Batch batch)
 {
-    std::unique_ptr<Ssimu2Batch> b;
-    {
-        std::lock_guard<std::mutex> g(g_lock);
-        if (batch.id >= g_batches.size() || !g_batches[batch.id]) {
-            g_last_error = Error(Vship_BadHandler, __FILE__, __LINE__, "invalid batch id in VshipAMD_SSIMU2BatchFree").message();
-            return Vship_BadHandler;
-        }
-        b = std::move(g_batches[batch.id]);
+    std::shared_ptr<Guarded<Ssimu2Batch>> b;
+    if (!g_batches.remove(batch.id, &b)) {
+        set_last(Error(Vship_BadHandler, __FILE__, __LINE__, "invalid batch id in VshipAMD_SSIMU2BatchFree").message());
+        return Vship_BadHandler;
     }
     return guarded([&] { b.reset(); });
 }
@@ -520,9 +492,10 @@ Vship_Exception VshipAMD_SSIMU2BatchCompute(VshipAMD_SSIMU2Batch batch, int npai
                                             const int64_t* lineSize2, double* scores)
 {
     return guarded([&] {
-        Ssimu2Batch* b = lookup_batch(batch.id);
-        if (!scores) VSHIP_THROW(Vship_BadPointer, "null scores");
-        b->compute(make_batch_input(npairs, src_cs, dis_cs, dsrcp1, dsrcp2, lineSize1, lineSize2), scores);
+        with_object(g_batches, batch.id, "VshipAMD_SSIMU2BatchCompute", [&](Ssimu2Batch& b) {
+            if (!scores) VSHIP_THROW(Vship_BadPointer, "null scores");
+            b.compute(make_batch_input(npairs, src_cs, dis_cs, dsrcp1, dsrcp2, lineSize1, lineSize2), scores);
+        });
     });
 }
 
@@ -533,9 +506,11 @@ Vship_Exception VshipAMD_SSIMU2BatchAutotune(VshipAMD_SSIMU2Batch batch, int npa
                                              double* best_ms)
 {
     return guarded([&] {
-        Ssimu2Batch* b = lookup_batch(batch.id);
         BatchTuning t;
-        b->autotune(make_batch_input(npairs, src_cs, dis_cs, dsrcp1, dsrcp2, lineSize1, lineSize2), reps, &t, best_ms);
+        with_object(g_batches, batch.id, "VshipAMD_SSIMU2BatchAutotune", [&](Ssimu2Batch& b) {
+            b.autotune(make_batch_input(npairs, src_cs, dis_cs, dsrcp1, dsrcp2, lineSize1, lineSize2), reps, &t,
+                       best_ms);
+        });
         if (chosen) {
             chosen->blur_config = t.blur_config;
             chosen->blur_waves = t.blur_waves;
@@ -551,7 +526,7 @@ Vship_Exception VshipAMD_SSIMU2BatchSetTuning(VshipAMD_SSIMU2Batch batch, VshipA
         t.blur_config = tuning.blur_config;
         t.blur_waves = tuning.blur_waves;
         t.front_waves = tuning.front_waves;
-        lookup_batch(batch.id)->set_tuning(t);
+        with_object(g_batches, batch.id, "VshipAMD_SSIMU2BatchSetTuning", [&](Ssimu2Batch& b) { b.set_tuning(t); });
     });
 }
 
@@ -559,7 +534,8 @@ Vship_Exception VshipAMD_SSIMU2BatchGetTuning(VshipAMD_SSIMU2Batch batch, VshipA
 {
     if (!tuning) return Vship_BadPointer;
     return guarded([&] {
-        const BatchTuning t = lookup_batch(batch.id)->tuning();
+        BatchTuning t;
+        with_object(g_batches, batch.id, "VshipAMD_SSIMU2BatchGetTuning", [&](Ssimu2Batch& b) { t = b.tuning(); });
         tuning->blur_config = t.blur_config;
         tuning->blur_waves = t.blur_waves;
         tuning->front_waves = t.front_waves;
@@ -569,20 +545,27 @@ Vship_Exception VshipAMD_SSIMU2BatchGetTuning(VshipAMD_SSIMU2Batch batch, VshipA
 Vship_Exception VshipAMD_SSIMU2BatchFeatures(VshipAMD_SSIMU2Batch batch, double* features)
 {
     if (!features) return Vship_BadPointer;
-    return guarded([&] { lookup_batch(batch.id)->features(features); });
+    return guarded([&] {
+        with_object(g_batches, batch.id, "VshipAMD_SSIMU2BatchFeatures", [&](Ssimu2Batch& b) { b.features(features); });
+    });
 }
 
 Vship_Exception VshipAMD_SSIMU2BatchProfile(VshipAMD_SSIMU2Batch batch, int iters, double* ms_frontend,
                                             double* ms_blur, double* ms_score, double* ms_graph)
 {
     if (!ms_frontend || !ms_blur || !ms_score || !ms_graph) return Vship_BadPointer;
-    return guarded([&] { lookup_batch(batch.id)->profile(iters, ms_frontend, ms_blur, ms_score, ms_graph); });
+    return guarded([&] {
+        with_object(g_batches, batch.id, "VshipAMD_SSIMU2BatchProfile",
+                    [&](Ssimu2Batch& b) { b.profile(iters, ms_frontend, ms_blur, ms_score, ms_graph); });
+    });
 }
 
 Vship_Exception VshipAMD_SSIMU2BatchWork(VshipAMD_SSIMU2Batch batch, double* bytes, double* flops)
 {
     if (!bytes || !flops) return Vship_BadPointer;
-    return guarded([&] { lookup_batch(batch.id)->work(bytes, flops); });
+    return guarded([&] {
+        with_object(g_batches, batch.id, "VshipAMD_SSIMU2BatchWork", [&](Ssimu2Batch& b) { b.work(bytes, flops); });
+    });
 }
 
 int VshipAMD_MLPParamCount(void) { return kMlpParams; }
@@ -594,25 +577,16 @@ Vship_Exception VshipAMD_MLPInit(VshipAMD_MLP* mlp, int max_batch, int gpu_id)
         const int n = device_count();
         if (gpu_id < 0 || gpu_id >= n) VSHIP_THROW(Vship_BadDeviceArgument, "In VshipAMD_MLPInit");
         std::unique_ptr<Surrogate> m(new Surrogate(max_batch, gpu_id));
-        std::lock_guard<std::mutex> g(g_lock);
-        unsigned id = 0;
-        while (id < g_mlps.size() && g_mlps[id]) id++;
-        if (id == g_mlps.size()) g_mlps.emplace_back();
-        g_mlps[id] = std::move(m);
-        mlp->id = id;
+        mlp->id = g_mlps.insert(std::make_shared<Guarded<Surrogate>>(std::move(m)));
     });
 }
 
 Vship_Exception VshipAMD_MLPFree(VshipAMD_MLP mlp)
 {
-    std::unique_ptr<Surrogate> m;
-    {
-        std::lock_guard<std::mutex> g(g_lock);
-        if (mlp.id >= g_mlps.size() || !g_mlps[mlp.id]) {
-            g_last_error = Error(Vship_BadHandler, __FILE__, __LINE__, "invalid surrogate id in VshipAMD_MLPFree").message();
-            return Vship_BadHandler;
-        }
-        m = std::move(g_mlps[mlp.id]);
+    std::shared_ptr<Guarded<Surrogate>> m;
+    if (!g_mlps.remove(mlp.id, &m)) {
+        set_last(Error(Vship_BadHandler, __FILE__, __LINE__, "invalid surrogate id in VshipAMD_MLPFree").message());
+        return Vship_BadHandler;
     }
     return guarded([&] { m.reset(); });
 }
@@ -621,7 +595,7 @@ Vship_Exception VshipAMD_MLPSetParams(VshipAMD_MLP mlp, const float* params, int
 {
     return guarded([&] {
         if (!params) VSHIP_THROW(Vship_BadPointer, "null parameter array");
-        lookup_mlp(mlp.id)->set_params(params, count);
+        with_object(g_mlps, mlp.id, "surrogate", [&](Surrogate& m) { m.set_params(params, count); });
     });
 }
 
@@ -629,7 +603,7 @@ Vship_Exception VshipAMD_MLPGetParams(VshipAMD_MLP mlp, float* params, int count
 {
     return guarded([&] {
         if (!params) VSHIP_THROW(Vship_BadPointer, "null parameter array");
-        lookup_mlp(mlp.id)->get_params(params, count);
+        with_object(g_mlps, mlp.id, "surrogate", [&](Surrogate& m) { m.get_params(params, count); });
     });
 }
 
@@ -637,7 +611,7 @@ Vship_Exception VshipAMD_MLPSetNorm(VshipAMD_MLP mlp, const float* xmean, const 
 {
     return guarded([&] {
         if (!xmean || !xstd) VSHIP_THROW(Vship_BadPointer, "null normalisation array");
-        lookup_mlp(mlp.id)->set_norm(xmean, xstd, ymean, ystd);
+        with_object(g_mlps, mlp.id, "surrogate", [&](Surrogate& m) { m.set_norm(xmean, xstd, ymean, ystd); });
     });
 }
 
@@ -645,7 +619,7 @@ Vship_Exception VshipAMD_MLPPredict(VshipAMD_MLP mlp, const float* dX, int n, fl
 {
     return guarded([&] {
         if (n > 0 && (!dX || !dmean || !dsigma)) VSHIP_THROW(Vship_BadPointer, "null sample array");
-        lookup_mlp(mlp.id)->predict(dX, n, dmean, dsigma);
+        with_object(g_mlps, mlp.id, "surrogate", [&](Surrogate& m) { m.predict(dX, n, dmean, dsigma); });
     });
 }
 
@@ -654,7 +628,7 @@ Vship_Exception VshipAMD_MLPGradients(VshipAMD_MLP mlp, const float* dX, const f
 {
     return guarded([&] {
         if (!dX || !dY || !grad) VSHIP_THROW(Vship_BadPointer, "null argument");
-        lookup_mlp(mlp.id)->gradients(dX, dY, n, grad, loss);
+        with_object(g_mlps, mlp.id, "surrogate", [&](Surrogate& m) { m.gradients(dX, dY, n, grad, loss); });
     });
 }
 
@@ -663,7 +637,7 @@ Vship_Exception VshipAMD_MLPTrain(VshipAMD_MLP mlp, const float* dX, const float
 {
     return guarded([&] {
         if (!dX || !dY) VSHIP_THROW(Vship_BadPointer, "null sample array");
-        lookup_mlp(mlp.id)->train(dX, dY, n, epochs, batch, lr, weight_decay, seed, loss_per_epoch);
+        with_object(g_mlps, mlp.id, "surrogate", [&](Surrogate& m) { m.train(dX, dY, n, epochs, batch, lr, weight_decay, seed, loss_per_epoch); });
     });
 }
 

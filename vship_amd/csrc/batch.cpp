@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "colorspace_key.hpp"
 #include "errors.hpp"
 #include "kernels.hpp"
 
@@ -73,7 +74,7 @@ Ssimu2Batch::~Ssimu2Batch()
 
 const ImageSpec& Ssimu2Batch::cached_spec(const Vship_Colorspace_t& cs)
 {
-    std::string key(reinterpret_cast<const char*>(&cs), sizeof(cs));
+    std::string key = colorspace_key(cs);
     auto it = spec_cache_.find(key);
     if (it != spec_cache_.end()) return it->second;
     if (spec_cache_.size() > 256) spec_cache_.clear();
@@ -126,7 +127,7 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
                     VSHIP_THROW(Vship_BadPointer, "lineSize smaller than one row of samples");
                 in_bytes += (double)specs[k]->plane_w[p] * specs[k]->plane_h[p] * specs[k]->bytes;
             }
-        std::string key(reinterpret_cast<const char*>(&in.src[i]), sizeof(Vship_Colorspace_t));
+        std::string key = colorspace_key(in.src[i]);
         key.append(reinterpret_cast<const char*>(pl[0]), 3 * sizeof(pl[0][0]));
         key.append(reinterpret_cast<const char*>(ls[0]), 3 * sizeof(ls[0][0]));
         const auto found = first_of.find(key);
@@ -223,7 +224,7 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
     xyb_elems_ = elems;
     front_tiles_ = ftiles;
     blur_tiles_ = btiles;
-    npairs_ = in.npairs;
+    table_pairs_ = in.npairs;
     // raw inputs read, stored planes written by the front-end and read twice
     // (both images) by the blur, 48 B of partials per 16x16 block
     work_bytes_ = in_bytes + (double)elem * (written + 2.0 * active) + 48.0 * blocks;
@@ -237,6 +238,7 @@ void Ssimu2Batch::ensure_capacity()
         (void)hipFree(xyb_);
         xyb_ = nullptr;
         xyb_cap_ = 0;
+        graph_valid_ = false;
         const int64_t want = xyb_elems_ + xyb_elems_ / 8;  // headroom for the next batches
         if (hipMalloc(&xyb_, (size_t)want * elem) != hipSuccess)
             VSHIP_THROW(Vship_OutOfVRAM, "batch XYB arena allocation failed");
@@ -268,7 +270,7 @@ void Ssimu2Batch::ensure_capacity()
     counts[kCountFrontJobs] = (int)fjobs_.size();
     counts[kCountBlurTiles] = blur_tiles_;
     counts[kCountBlurJobs] = (int)dev_maps.size();
-    counts[kCountPairs] = npairs_;
+    counts[kCountPairs] = table_pairs_;
     counts[kCountFrontShared] = front_shared_;
     auto put = [&](const void* dev, const void* src, size_t bytes) {
         std::memcpy(h_tables_ + (static_cast<const uint8_t*>(dev) - tables_), src, bytes);
@@ -324,17 +326,22 @@ void Ssimu2Batch::capture()
 
 void Ssimu2Batch::compute(const BatchInput& in, double* scores)
 {
+    // features() and profile() refer to the last call that completed: none
+    // until this one has
+    npairs_ = 0;
     VSHIP_HIP_CHECK(hipSetDevice(device_));
     build_tables(in);
     ensure_capacity();
     if (!graph_valid_) capture();
     VSHIP_HIP_CHECK(hipGraphLaunch(exec_, stream_));
     VSHIP_HIP_CHECK(hipStreamSynchronize(stream_));
+    npairs_ = table_pairs_;
     std::memcpy(scores, h_scores_, (size_t)npairs_ * sizeof(double));
 }
 
 void Ssimu2Batch::features(double* out)
 {
+    if (npairs_ == 0) VSHIP_THROW(Vship_BadPointer, "features need one successful compute call first");
     VSHIP_HIP_CHECK(hipSetDevice(device_));
     VSHIP_HIP_CHECK(hipMemcpy(out, d_features_, (size_t)npairs_ * 108 * sizeof(double), hipMemcpyDeviceToHost));
 }
@@ -350,6 +357,7 @@ void Ssimu2Batch::set_tuning(const BatchTuning& t)
 
 void Ssimu2Batch::autotune(const BatchInput& in, int reps, BatchTuning* chosen, double* best_ms)
 {
+    npairs_ = 0;  // the replays leave features of no completed compute
     VSHIP_HIP_CHECK(hipSetDevice(device_));
     reps = std::max(1, reps);
     hipEvent_t e0, e1;
@@ -395,7 +403,7 @@ void Ssimu2Batch::autotune(const BatchInput& in, int reps, BatchTuning* chosen, 
 
 void Ssimu2Batch::profile(int iters, double* ms_front, double* ms_blur, double* ms_score, double* ms_graph)
 {
-    if (npairs_ == 0) VSHIP_THROW(Vship_BadPointer, "profile needs one compute call first");
+    if (npairs_ == 0) VSHIP_THROW(Vship_BadPointer, "profile needs one successful compute call first");
     VSHIP_HIP_CHECK(hipSetDevice(device_));
     iters = std::max(1, iters);
     hipEvent_t ev[4];

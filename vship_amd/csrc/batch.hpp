@@ -93,7 +93,9 @@ private:
     const ImageSpec& cached_spec(const Vship_Colorspace_t& cs);
     const PairLayout& cached_layout(int w, int h, int blur_config);
     int64_t xyb_elems_ = 0;
-    int front_tiles_ = 0, blur_tiles_ = 0, npairs_ = 0;
+    int front_tiles_ = 0, blur_tiles_ = 0;
+    int table_pairs_ = 0;  // pairs of the tables built last
+    int npairs_ = 0;       // pairs of the last compute that completed, 0 if none
     double work_bytes_ = 0.0, work_flops_ = 0.0;
 
     // device
