@@ -25,6 +25,25 @@ EXPORTPREPROCESS Vship_Exception VshipAMD_ComputeSSIMU2Device(Vship_SSIMU2Handle
                                                               const uint8_t* dsrcp1[3], const uint8_t* dsrcp2[3],
                                                               const int64_t lineSize[3], const int64_t lineSize2[3]);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the score of one pair of device-resident planes with no
+ * handler and no allocation. The same kernels and host completion as
+ * VshipAMD_ComputeSSIMU2Device (same score, bit for bit), but the tables, both
+ * images' FP32 XYB planes and the partial sums live in a fixed 32 MiB device
+ * array that each GPU the library runs on holds until the process ends. Pairs
+ * that need more (above about 2.3 Mpixel, e.g. 2560x1440; 1920x1080 and
+ * 2048x1080 fit) or more than 1024 blur items return Vship_OutOfVRAM, crop or
+ * resize Vship_NonRGBSInput. Runs on gpu_id's null stream and blocks until
+ * *score is written; uses about 40 KB of the calling thread's stack. Calls that
+ * overlap on one GPU share its array and race: serialise them. */
+EXPORTPREPROCESS Vship_Exception VshipAMD_ComputeSSIMU2DeviceNoAlloc(Vship_Colorspace_t src_colorspace,
+                                                                     Vship_Colorspace_t dis_colorspace, double* score,
+                                                                     const uint8_t* dsrcp1[3],
+                                                                     const uint8_t* dsrcp2[3],
+                                                                     const int64_t lineSize[3],
+                                                                     const int64_t lineSize2[3], int gpu_id);
+
 /* The 108 features (index plane*36 + scale*6 + norm*3 + map) of the last
  * compute on this handler; features of maps the score does not use are 0. */
 EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2Features(Vship_SSIMU2Handler handler, double features[108]);
@@ -83,7 +102,7 @@ EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2BatchFree(VshipAMD_SSIMU2Batch b
  * batch's GPU. Pair i uses src_cs[i] / dis_cs[i] (any sizes, the two of a
  * pair equal), planes dsrcp1[3i..3i+2] / dsrcp2[3i..3i+2] and line sizes
  * lineSize1[3i..] / lineSize2[3i..]. Blocks until scores[0..npairs) are
- * written. Pairs whose reference (src_cs bytes, the three dsrcp1 pointers and
+ * written. Pairs whose reference (the src_cs fields, the three dsrcp1 pointers and
  * line sizes) repeats an earlier pair's reuse that pair's stored reference
  * pyramid (candidate screening); the results are the same bit for bit. */
 EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2BatchCompute(VshipAMD_SSIMU2Batch batch, int npairs,

@@ -165,3 +165,18 @@ def test_free_from_a_second_thread_while_computing():
     for i in ids:
         lib.Vship_SSIMU2Free(S.Handler(i))
     assert codes and codes <= {6, 7, 10, 11}, codes
+
+
+def test_noalloc_entry_point_rejects_on_the_host():
+    # frames beyond the no-allocation entry point's fixed scratch array, and
+    # crop/resize, are refused before any device is touched (also without a GPU)
+    dummy = [1 << 20] * 3
+    cs = S.colorspace(2560, 1440)
+    with pytest.raises(S.VshipError) as e:
+        S.compute_device_noalloc(cs, cs, dummy, dummy, [2560, 1280, 1280], [2560, 1280, 1280])
+    assert e.value.code == 1 and "at most 33554432 bytes" in str(e.value)
+    cs = S.colorspace(640, 360)
+    cs.target_width = 320
+    with pytest.raises(S.VshipError) as e:
+        S.compute_device_noalloc(cs, cs, dummy, dummy, [640, 320, 320], [640, 320, 320])
+    assert e.value.code == 5

@@ -26,6 +26,7 @@
 #include "handle_table.hpp"
 #include "kernels.hpp"
 #include "mlp.hpp"
+#include "noalloc.hpp"
 
 namespace vship {
 
@@ -426,6 +427,17 @@ Vship_Exception VshipAMD_ComputeSSIMU2Device(Vship_SSIMU2Handler handler, double
     return handler_call(handler.id, "VshipAMD_ComputeSSIMU2Device", [&](Ssimu2Handler& h) {
         if (!score || !dsrcp1 || !dsrcp2 || !lineSize || !lineSize2) VSHIP_THROW(Vship_BadPointer, "null argument");
         *score = h.compute_device(dsrcp1, dsrcp2, lineSize, lineSize2);
+    });
+}
+
+Vship_Exception VshipAMD_ComputeSSIMU2DeviceNoAlloc(Vship_Colorspace_t src_colorspace,
+                                                    Vship_Colorspace_t dis_colorspace, double* score,
+                                                    const uint8_t* dsrcp1[3], const uint8_t* dsrcp2[3],
+                                                    const int64_t lineSize[3], const int64_t lineSize2[3], int gpu_id)
+{
+    return guarded([&] {
+        if (!score || !dsrcp1 || !dsrcp2 || !lineSize || !lineSize2) VSHIP_THROW(Vship_BadPointer, "null argument");
+        *score = compute_ssimu2_noalloc(src_colorspace, dis_colorspace, dsrcp1, dsrcp2, lineSize, lineSize2, gpu_id);
     });
 }
 

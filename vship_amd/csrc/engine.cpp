@@ -307,6 +307,12 @@ BlurTaps make_blur_taps()
     return t;
 }
 
+void MapJobList::push_back(const MapJob& m)
+{
+    if (count >= kMaxJobsPerPair) VSHIP_THROW(Vship_BadErrorType, "more map jobs than a pair has");
+    items[count++] = m;
+}
+
 PairLayout make_pair_layout(int w, int h, int blur_config)
 {
     const BlurTile bt = blur_tile_shape(blur_config);
@@ -399,7 +405,7 @@ Ssimu2Handler::Ssimu2Handler(const Vship_Colorspace_t& src, const Vship_Colorspa
         VSHIP_THROW(Vship_OutOfRAM, "hipHostMalloc failed in SSIMU2 init");
 
     // static tables
-    std::vector<MapJob> maps = layout_.maps;
+    MapJobList maps = layout_.maps;
     for (auto& m : maps) {
         const int64_t off = reinterpret_cast<int64_t>(m.a);
         m.a = static_cast<const float*>(xyb_[0]) + off;
@@ -532,12 +538,13 @@ static double butterfly_lane(const double v[64], int lane)
     }
 }
 
-static double host_score(const std::vector<MapJob>& maps, const float* partials, double feat[108])
+double score_from_partials(const MapJob* maps, int nmaps, const float* partials, double feat[108])
 {
 #pragma clang fp contract(off)
     static const double kWeights[108] = VSHIP_SSIMU2_WEIGHTS;
     for (int i = 0; i < 108; i++) feat[i] = 0.0;
-    for (const MapJob& job : maps) {
+    for (int jn = 0; jn < nmaps; jn++) {
+        const MapJob& job = maps[jn];
         double acc[6][64] = {};
         for (int k = 0; k < job.ntiles; k++) {
             const float* p = partials + (size_t)(job.tile_start + k) * kPartialStride;
@@ -584,7 +591,7 @@ void Ssimu2Handler::launch_all()
     VSHIP_HIP_CHECK(launch_frontend_single<float>(*h_front_jobs_, layout_.front_tiles_per_image, d_colors_, d_geom_,
                                                   call_spec(), stream_));
     // the items' partial sums go straight to pinned host memory; run() forms
-    // the score from them (host_score), so a call is two kernel launches
+    // the score from them (score_from_partials), so a call is two kernel launches
     VSHIP_HIP_CHECK(launch_blur<float>(layout_.blur_config, d_maps_, d_counts_, blur_grid_for(layout_.blur_tiles),
                                        taps_, h_partials_, stream_));
 }
@@ -606,7 +613,7 @@ void Ssimu2Handler::run(const uint8_t* const p1[3], const uint8_t* const p2[3], 
     }
     launch_all();
     VSHIP_HIP_CHECK(hipStreamSynchronize(stream_));
-    *h_score_ = host_score(layout_.maps, h_partials_, h_features_);
+    *h_score_ = score_from_partials(layout_.maps.data(), layout_.maps.count, h_partials_, h_features_);
     have_run_ = true;
 }
 
@@ -683,7 +690,7 @@ void Ssimu2Handler::profile(int iters, double* ms_front, double* ms_blur, double
         VSHIP_HIP_CHECK(hipEventSynchronize(ev[2]));
         // the score's FP64 completion on the host, as in run()
         const auto t0 = std::chrono::steady_clock::now();
-        (void)host_score(layout_.maps, h_partials_, feat);
+        (void)score_from_partials(layout_.maps.data(), layout_.maps.count, h_partials_, feat);
         acc[2] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         for (int k = 0; k < 2; k++) {
             float ms = 0.0f;

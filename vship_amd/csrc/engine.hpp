@@ -42,18 +42,39 @@ ImageSpec make_image_spec(const Vship_Colorspace_t& cs);
 // (src/ssimu2/gaussianblur.hpp:7-14): float values and float prefix sums.
 BlurTaps make_blur_taps();
 
+// The map jobs of one pair (at most kMaxJobsPerPair), held inline so that
+// building a layout allocates nothing.
+struct MapJobList {
+    MapJob items[kMaxJobsPerPair];
+    int count = 0;
+    void push_back(const MapJob& m);
+    size_t size() const { return (size_t)count; }
+    const MapJob* data() const { return items; }
+    MapJob* data() { return items; }
+    const MapJob* begin() const { return items; }
+    const MapJob* end() const { return items + count; }
+    MapJob* begin() { return items; }
+    MapJob* end() { return items + count; }
+};
+
 // Geometry and map jobs of one pair of size w x h; xyb offsets relative to the
 // start of each image's XYB buffer.
 struct PairLayout {
     PairGeom geom;
     int front_tiles_per_image;
     int front_tiles_x;
-    std::vector<MapJob> maps;     // a/b filled with element offsets, not pointers
+    MapJobList maps;              // a/b filled with element offsets, not pointers
     int blur_tiles;
     int blur_config;
 };
 
 PairLayout make_pair_layout(int w, int h, int blur_config = kDefaultBlurConfig);
+
+// FP64 completion of one pair's score from its blur items' partial sums
+// (kPartialStride floats per item, items numbered by MapJob::tile_start):
+// the 108 features and the score, with the operations and order of the GPU's
+// score kernel (so both paths give the same features).
+double score_from_partials(const MapJob* maps, int nmaps, const float* partials, double features[108]);
 
 class Ssimu2Handler {
 public:

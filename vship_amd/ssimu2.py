@@ -157,6 +157,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "Vship_ComputeSSIMU2": (E, [Handler, P(ctypes.c_double), u8pp, u8pp, i64p, i64p]),
         "Vship_SSIMU2GetDetailedLastError": (ctypes.c_int, [Handler, ctypes.c_char_p, ctypes.c_int]),
         "VshipAMD_ComputeSSIMU2Device": (E, [Handler, P(ctypes.c_double), u8pp, u8pp, i64p, i64p]),
+        "VshipAMD_ComputeSSIMU2DeviceNoAlloc": (E, [Colorspace, Colorspace, P(ctypes.c_double), u8pp, u8pp, i64p,
+                                                    i64p, ctypes.c_int]),
         "VshipAMD_SSIMU2Features": (E, [Handler, P(ctypes.c_double)]),
         "VshipAMD_SSIMU2DebugPlane": (E, [Handler, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_float),
                                           i64p, i64p]),
@@ -355,6 +357,18 @@ class SSIMU2Handler:
             self.close()
         except Exception:
             pass
+
+
+def compute_device_noalloc(src: Colorspace, dis: Colorspace, dptrs1: Sequence[int], dptrs2: Sequence[int],
+                           strides1: Sequence[int], strides2: Sequence[int], gpu_id: int = 0) -> float:
+    """VshipAMD_ComputeSSIMU2DeviceNoAlloc: the score of one pair of device-resident planes without a
+    handler and without allocating device memory (the library's fixed per-GPU scratch array)."""
+    lib = load_library()
+    p1, l1 = _plane_args(dptrs1, strides1)
+    p2, l2 = _plane_args(dptrs2, strides2)
+    score = ctypes.c_double(0.0)
+    check(lib.VshipAMD_ComputeSSIMU2DeviceNoAlloc(src, dis, ctypes.byref(score), p1, p2, l1, l2, gpu_id))
+    return score.value
 
 
 STORAGE_FP32, STORAGE_FP16 = 0, 1
