@@ -1,7 +1,8 @@
 """FP64 CPU oracle of the SSIMULACRA2 path (test infrastructure only).
 
-ctypes wrapper of oracle/liboracle.so (built from oracle/ssimu2_oracle.c by
-oracle/Makefile). Only tests/, __graft_entry__.smoke() and the cpu_baseline
+ctypes wrapper of oracle/liboracle.so (built from oracle/ssimu2_oracle.c, the
+exact formula in FP64, and oracle/ref_float.c, an emulation of the
+reference's own FP32 rounding, by oracle/Makefile). Only tests/, __graft_entry__.smoke() and the cpu_baseline
 leg of bench.py use it, as the checker; the product never imports it.
 """
 from __future__ import annotations
@@ -57,6 +58,13 @@ def lib() -> ctypes.CDLL:
         L.oracle_final_score.argtypes = [_D]
         L.oracle_features_from_linear.argtypes = [_D, _D, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, _D]
         L.oracle_ssimu2.argtypes = [CS, vp3, i64, CS, vp3, i64, ctypes.c_int, _D, _D]
+        L.oracle_reference_float.argtypes = [CS, vp3, i64, CS, vp3, i64, _D, _D]
+        _F = ctypes.POINTER(ctypes.c_float)
+        L.oracle_reference_float_from_linear.restype = ctypes.c_double
+        L.oracle_reference_float_from_linear.argtypes = [_F, _F, ctypes.c_int64, ctypes.c_int64, _D]
+        L.oracle_reference_float_perturbed.restype = ctypes.c_double
+        L.oracle_reference_float_perturbed.argtypes = [_F, _F, ctypes.c_int64, ctypes.c_int64, ctypes.c_uint64,
+                                                       ctypes.c_double]
         _lib = L
     return _lib
 
@@ -201,3 +209,37 @@ def features_from_linear(rgb1: np.ndarray, rgb2: np.ndarray, prune: bool = True)
     if rc != 0:
         raise ValueError("oracle_features_from_linear failed")
     return f
+
+
+def reference_float(cs1, planes1, strides1, cs2, planes2, strides2):
+    """(score, features[108]) of the emulation of the reference's FP32 arithmetic (oracle/ref_float.c):
+    what the reference libvship returns up to the float rounding of its colour front-end."""
+    f = np.zeros(108)
+    s = ctypes.c_double()
+    p1, l1 = _planes(planes1, strides1)
+    p2, l2 = _planes(planes2, strides2)
+    rc = lib().oracle_reference_float(ctypes.byref(cs1), p1, l1, ctypes.byref(cs2), p2, l2, _dp(f), ctypes.byref(s))
+    if rc != 0:
+        raise ValueError(f"oracle_reference_float failed ({rc})")
+    return s.value, f
+
+
+def reference_float_from_linear(rgb1: np.ndarray, rgb2: np.ndarray):
+    """The emulated reference score of two (3, h, w) linear-RGB float32 images."""
+    a = np.ascontiguousarray(rgb1, dtype=np.float32)
+    b = np.ascontiguousarray(rgb2, dtype=np.float32)
+    f = np.zeros(108)
+    fp = ctypes.POINTER(ctypes.c_float)
+    s = lib().oracle_reference_float_from_linear(a.ctypes.data_as(fp), b.ctypes.data_as(fp), a.shape[2], a.shape[1],
+                                                 _dp(f))
+    return s, f
+
+
+def reference_float_spread(rgb1: np.ndarray, rgb2: np.ndarray, draws: int = 16, frac: float = 0.5):
+    """Emulated reference scores with a fraction `frac` of the XYB values moved by one ulp, one per seed:
+    the spread is how far last-bit differences of the reference's libm can move its score."""
+    a = np.ascontiguousarray(rgb1, dtype=np.float32)
+    b = np.ascontiguousarray(rgb2, dtype=np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    return np.array([lib().oracle_reference_float_perturbed(a.ctypes.data_as(fp), b.ctypes.data_as(fp), a.shape[2],
+                                                            a.shape[1], 1000 + k, frac) for k in range(draws)])

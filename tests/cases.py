@@ -87,6 +87,31 @@ def perturbed_pair(cs, seed: int, pad: int = 0, noise: float = 0.025):
     return ref, dist, strides
 
 
+def smooth_pair(cs, seed: int, pad: int = 0):
+    """Natural-like, well-conditioned reference planes and a compressed copy.
+
+    Per plane a sum of 32 random sinusoids (the workload generator's) scaled
+    into nominal luma (or R, G, B) [0.2, 0.8] / chroma [-0.06, 0.06]; the
+    distorted plane is its 8x8 block DCT quantisation (step 12 on an 8-bit
+    scale) plus N(0, 0.002^2). Rows are padded by pad*(plane+1) bytes.
+    """
+    rng = np.random.default_rng(seed)
+    yuv = cs.colorFamily == S.FAMILY_YUV
+    ref_nom, dist_nom = [], []
+    for p in range(3):
+        h, w = cs.plane_shape(p)
+        chroma = yuv and p > 0
+        f = synth._sinusoid_sum(h, w, rng)
+        f = (f - f.min()) / max(f.max() - f.min(), 1e-9) * 2.0 - 1.0
+        v = 0.06 * f if chroma else 0.5 + 0.3 * f
+        d = synth.dct_quantize(v * 255.0, 12.0) / 255.0 + rng.normal(0, 0.002, (h, w))
+        ref_nom.append((v, chroma))
+        dist_nom.append((d, chroma))
+    ref, strides = _pack(cs, [_code_values(cs, v, c) for v, c in ref_nom], pad)
+    dist, _ = _pack(cs, [_code_values(cs, v, c) for v, c in dist_nom], pad)
+    return ref, dist, strides
+
+
 def out_of_gamut_pair(cs, seed: int):
     """Uniform random samples over the whole code range (negative and >1
     linear light included) and a perturbed copy."""

@@ -29,3 +29,24 @@ def golden_inputs():
         c = S.colorspace(131, 75, **kw)
         r, d, st = cases.perturbed_pair(c, seed=3, pad=5)
         yield f"random131x75_{name}", c, r, st, c, d, st
+    # the 16 frames of the throughput comparison with the reference
+    seq = synth.YuvSequence(1920, 1080, frames=16, seed=7)
+    c = S.colorspace(1920, 1080)
+    for i in range(16):
+        r, d, _ = seq.frame(i)
+        st = S.host_strides(r)
+        yield f"yuv420p8_1920x1080_seed7_frame{i}", c, r, st, c, d, st
+    # every colourspace on smooth (well-conditioned) content
+    for name, kw in cases.COLOR_CASES:
+        c = S.colorspace(320, 180, **kw)
+        r, d, st = cases.smooth_pair(c, seed=5, pad=5)
+        yield f"smooth320x180_{name}", c, r, st, c, d, st
+    # the ST240 case's ingredients one at a time, on its random content
+    for name, kw, pad in (("yuv420p14_st240_dense", dict(sample=S.SAMPLE_UINT14, trc=S.TRC_ST240_M,
+                                                           prim=S.PRIM_ST240_M), 0),
+                          ("yuv420p14_709", dict(sample=S.SAMPLE_UINT14), 5),
+                          ("yuv420p8_st240trc", dict(trc=S.TRC_ST240_M), 5),
+                          ("yuv420p8_st240prim", dict(prim=S.PRIM_ST240_M), 5)):
+        c = S.colorspace(131, 75, **kw)
+        r, d, st = cases.perturbed_pair(c, seed=3, pad=pad)
+        yield f"random131x75_{name}", c, r, st, c, d, st

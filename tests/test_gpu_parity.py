@@ -1,8 +1,9 @@
 """GPU parity of the HIP path (through the C ABI) against the FP64 oracle and,
 where it was built, the reference libvship on the same inputs.
 
-Bars: |delta score| <= 0.005 against the oracle and the reference, identical
-inputs give exactly 100, stored XYB planes within 5e-5 of the oracle's.
+Bars: |delta score| <= 0.005 against the oracle; against the reference, its
+own float error as tests/test_golden.py measures it; identical inputs give
+exactly 100; stored XYB planes within 5e-5 of the oracle's.
 """
 import zlib
 
@@ -221,31 +222,45 @@ def test_error_paths_on_gpu():
 def test_against_reference_libvship():
     """GPU vs the reference libvship vs the FP64 oracle on the golden inputs.
 
-    On the synthetic natural-content frames (sRGB 256x256, YUV420 360p..1080p)
-    all three agree within 0.005. On 131x75 uniform-random planes the
-    reference's own float error against the oracle reaches ~0.02 (E[x^2] - mu^2
-    cancels at the smallest scales); there the bar is the GPU against the
-    oracle, and the GPU may differ from the reference by that reference error.
+    The GPU path is held to the FP64 oracle (0.005 everywhere). Against
+    libvship the bar is the reference's own float error, which
+    oracle/ref_float.c reproduces (tests/test_golden.py): up to 0.0084 on
+    natural content (bar 0.01), and on the small random and high-contrast
+    colourspace cases 0.002 + 6 sigma of the reference formula's libm noise,
+    measured per case, plus the 0.005 GPU tolerance.
     """
+    import concurrent.futures as cf
+
     from tests.golden.inputs import golden_inputs
+    from tests.test_golden import _colourspace_case, _noise_sigma
+    items = list(golden_inputs())
+    _say(f"golden inputs: {len(items)}")
+
+    def exact(item):
+        name, cs1, p1, s1, cs2, p2, s2 = item
+        score, _ = oracle.ssimu2(cs1, p1, s1, cs2, p2, s2)
+        return score, (_noise_sigma(item) if _colourspace_case(name) else 0.0)
+
+    with cf.ThreadPoolExecutor(max_workers=8) as ex:
+        oracle_scores = list(ex.map(exact, items))
     ref_lib = refvship.RefVship()
-    for name, cs1, p1, s1, cs2, p2, s2 in golden_inputs():
+    for item, (want_or, sigma) in zip(items, oracle_scores):
+        name, cs1, p1, s1, cs2, p2, s2 = item
         want_ref = ref_lib.score(cs1, cs2, p1, s1, p2, s2)
-        want_or, _ = oracle.ssimu2(cs1, p1, s1, cs2, p2, s2)
         with S.SSIMU2Handler(cs1, cs2) as hd:
             got = hd.compute(p1, p2, s1, s2)
         _say(f"ref {name}: gpu {got:.6f} reference {want_ref:.6f} oracle {want_or:.6f} "
-             f"gpu-oracle {got - want_or:+.2e} ref-oracle {want_ref - want_or:+.2e}")
+             f"gpu-oracle {got - want_or:+.2e} gpu-ref {got - want_ref:+.2e}")
         assert abs(got - want_or) <= SCORE_TOL, name
         if name.endswith("identity"):
             # the reference's SSIM numerator and denominator round differently
             # for a == b, so it scores identical frames below 100; its own
             # identity test expects exactly 100, which this library returns
             assert got == 100.0 and want_or == 100.0
-        elif name.startswith("random"):
-            assert abs(got - want_ref) <= abs(want_ref - want_or) + SCORE_TOL, name
+        elif _colourspace_case(name):
+            assert abs(got - want_ref) <= 0.002 + 6.0 * sigma + SCORE_TOL, (name, sigma)
         else:
-            assert abs(got - want_ref) <= SCORE_TOL, name
+            assert abs(got - want_ref) <= 0.01, name
 
 
 RESAMPLE_CASES = [
