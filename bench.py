@@ -181,8 +181,9 @@ def mixed_leg(npairs: int, steps: int, warmup: int):
 def screening_leg(steps: int, target: float = 70.0):
     """Target-quality screening: 64 1080p YUV420 sources x 16 DCT candidates
     (q from 2 to 64, geometric). Exact-only selection (every candidate
-    through the batched FP16 metric) against surrogate screening (network on
-    the scale 0-2 features, exact confirmation of the top 4 per source, exact
+    through the batched FP16 metric) against surrogate screening (one pass
+    over scales 0-2 of every candidate for the network's inputs, exact
+    confirmation of the top 4 per source by completing their scales 3-4, exact
     fallback for uncertain candidates)."""
     from vship_amd import surrogate as SG
     if not os.path.exists(SG.PARAMS_FILE):
@@ -208,6 +209,7 @@ def screening_leg(steps: int, target: float = 70.0):
     finally:
         scr.close()
     agree = float(np.mean([a == b for a, b in zip(exact_choice, sur_choice)]))
+    confirmed_equal = all(v == exact_scores[i, k] for (i, k), v in stats["confirmed"].items())
     err = np.abs(mean - exact_scores)
     log(f"screening: exact-only {ncand / t_exact:.0f} candidates/s, surrogate {ncand / t_sur:.0f} candidates/s, "
         f"agreement {agree:.3f}")
@@ -217,6 +219,7 @@ def screening_leg(steps: int, target: float = 70.0):
         "exact_only_candidates_per_s": round(ncand / t_exact, 1),
         "surrogate_candidates_per_s": round(ncand / t_sur, 1),
         "agreement_with_exact_only": agree,
+        "confirmed_scores_equal_exact_only": confirmed_equal,
         "confirmed_pairs": stats["confirmed_pairs"],
         "uncertain_pairs_sent_to_exact": stats["uncertain_pairs_sent_to_exact"],
         "sources_fully_rescored": stats["sources_fully_rescored"],

@@ -112,6 +112,26 @@ EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2BatchCompute(VshipAMD_SSIMU2Batc
                                                              const int64_t* lineSize1, const int64_t* lineSize2,
                                                              double* scores);
 
+/* Two-step scoring for candidate screening. ComputeScales is BatchCompute
+ * with only the map jobs of the scales in scale_mask blurred (bit s = scale s
+ * of the pyramid; 0x1f = scales 0-4, all the score uses). Every plane is
+ * still stored; the features of the other scales are 0 and scores[] are
+ * those of the selected scales only. CompleteScales then blurs the scales in
+ * its scale_mask for `count` pairs of that call (indices into it, each at
+ * most once) on the planes it stored and writes their full scores (and
+ * features, see BatchFeatures); the two masks must together cover 0x1f and
+ * no other compute may run on the batch in between. The completed scores and
+ * features equal BatchCompute's bit for bit. */
+EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2BatchComputeScales(VshipAMD_SSIMU2Batch batch, int npairs,
+                                                                   const Vship_Colorspace_t* src_cs,
+                                                                   const Vship_Colorspace_t* dis_cs,
+                                                                   const uint8_t* const* dsrcp1,
+                                                                   const uint8_t* const* dsrcp2,
+                                                                   const int64_t* lineSize1, const int64_t* lineSize2,
+                                                                   int scale_mask, double* scores);
+EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2BatchCompleteScales(VshipAMD_SSIMU2Batch batch, int count,
+                                                                    const int* pairs, int scale_mask, double* scores);
+
 /* Times every tuning candidate on the given pairs (reps launches each) and
  * keeps the fastest; returns it and its mean ms per batch. */
 EXPORTPREPROCESS Vship_Exception VshipAMD_SSIMU2BatchAutotune(VshipAMD_SSIMU2Batch batch, int npairs,

@@ -377,3 +377,35 @@ def test_failed_compute_leaves_no_stale_results(pairs):
         got = b.compute(*pairs.args())
         assert np.array_equal(got, want)
         assert b.features(n).shape == (n, 108)
+
+
+SCALE_OF = np.array([(i % 36) // 6 for i in range(108)])
+
+
+@pytest.mark.parametrize("storage", [S.STORAGE_FP32, S.STORAGE_FP16], ids=["fp32", "fp16"])
+def test_two_step_scales_equal_a_full_compute(pairs, storage):
+    """ComputeScales(scales 0-2) then CompleteScales(scales 3-4) of chosen
+    pairs gives the full compute's scores and features bit for bit: the same
+    items land in the same partial-sum slots."""
+    n = len(SIZES)
+    with S.SSIMU2Batch(16, storage) as b:
+        full = b.compute(*pairs.args())
+        ffeat = b.features(n)
+        args = b.prepare(*pairs.args())
+        b.compute_prepared(args, np.zeros(n), 0b00111)
+        pf = b.features(n)
+        assert np.array_equal(pf[:, SCALE_OF <= 2], ffeat[:, SCALE_OF <= 2])
+        assert not pf[:, SCALE_OF >= 3].any()
+        idx = [1, 5, 2]
+        got = b.complete_scales(idx, 0b11000)
+        _say("two-step", storage, got, full[idx])
+        assert np.array_equal(got, full[idx])
+        assert np.array_equal(b.features(n)[idx], ffeat[idx])
+        rest = [i for i in range(n) if i not in idx]
+        assert np.array_equal(b.complete_scales(rest, 0b11000), full[rest])
+        assert np.array_equal(b.features(n), ffeat)
+        # rejected on the host: masks that leave a scale out, a pair twice, an index out of range
+        for bad_idx, mask in (([0], 0b01000), ([0, 0], 0b11000), ([n], 0b11000), ([0], 0)):
+            with pytest.raises(S.VshipError) as e:
+                b.complete_scales(bad_idx, mask)
+            assert e.value.code == 11

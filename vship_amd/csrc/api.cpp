@@ -511,6 +511,30 @@ Vship_Exception VshipAMD_SSIMU2BatchCompute(VshipAMD_SSIMU2Batch batch, int npai
     });
 }
 
+Vship_Exception VshipAMD_SSIMU2BatchComputeScales(VshipAMD_SSIMU2Batch batch, int npairs,
+                                                  const Vship_Colorspace_t* src_cs, const Vship_Colorspace_t* dis_cs,
+                                                  const uint8_t* const* dsrcp1, const uint8_t* const* dsrcp2,
+                                                  const int64_t* lineSize1, const int64_t* lineSize2, int scale_mask,
+                                                  double* scores)
+{
+    return guarded([&] {
+        with_object(g_batches, batch.id, "VshipAMD_SSIMU2BatchComputeScales", [&](Ssimu2Batch& b) {
+            if (!scores) VSHIP_THROW(Vship_BadPointer, "null scores");
+            b.compute(make_batch_input(npairs, src_cs, dis_cs, dsrcp1, dsrcp2, lineSize1, lineSize2), scores,
+                      scale_mask);
+        });
+    });
+}
+
+Vship_Exception VshipAMD_SSIMU2BatchCompleteScales(VshipAMD_SSIMU2Batch batch, int count, const int* pairs,
+                                                   int scale_mask, double* scores)
+{
+    return guarded([&] {
+        with_object(g_batches, batch.id, "VshipAMD_SSIMU2BatchCompleteScales",
+                    [&](Ssimu2Batch& b) { b.complete(count, pairs, scale_mask, scores); });
+    });
+}
+
 Vship_Exception VshipAMD_SSIMU2BatchAutotune(VshipAMD_SSIMU2Batch batch, int npairs, const Vship_Colorspace_t* src_cs,
                                              const Vship_Colorspace_t* dis_cs, const uint8_t* const* dsrcp1,
                                              const uint8_t* const* dsrcp2, const int64_t* lineSize1,

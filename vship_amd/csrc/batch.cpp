@@ -17,6 +17,9 @@ constexpr int kScoreGrid = 256;
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// scale of a map job, from its feature index plane*36 + scale*6
+int job_scale(const MapJob& m) { return (m.feature_base % 36) / 6; }
+
 }  // namespace
 
 Ssimu2Batch::Ssimu2Batch(int max_pairs, bool fp16, int device)
@@ -39,7 +42,13 @@ Ssimu2Batch::Ssimu2Batch(int max_pairs, bool fp16, int device)
     const size_t o_colors = off; off = align_up(off + 2 * cap_fjobs_ * sizeof(FrontColor), 256);
     const size_t o_geoms = off; off = align_up(off + max_pairs * sizeof(PairGeom), 256);
     const size_t o_maps = off; off = align_up(off + cap_maps_ * sizeof(MapJob), 256);
+    const size_t o_bmaps = off; off = align_up(off + cap_maps_ * sizeof(MapJob), 256);
     const size_t o_sjobs = off; off = align_up(off + max_pairs * sizeof(ScoreJob), 256);
+    // the completion's tables
+    const size_t o_ccounts = off; off = align_up(off + kCountSlots * sizeof(int), 256);
+    const size_t o_cmaps = off; off = align_up(off + cap_maps_ * sizeof(MapJob), 256);
+    const size_t o_cbmaps = off; off = align_up(off + cap_maps_ * sizeof(MapJob), 256);
+    const size_t o_csjobs = off; off = align_up(off + max_pairs * sizeof(ScoreJob), 256);
     tables_cap_ = off;
     if (hipMalloc((void**)&tables_, tables_cap_) != hipSuccess ||
         hipMalloc((void**)&d_features_, (size_t)max_pairs * 108 * sizeof(double)) != hipSuccess ||
@@ -53,7 +62,12 @@ Ssimu2Batch::Ssimu2Batch(int max_pairs, bool fp16, int device)
     d_colors_ = reinterpret_cast<FrontColor*>(tables_ + o_colors);
     d_geoms_ = reinterpret_cast<PairGeom*>(tables_ + o_geoms);
     d_maps_ = reinterpret_cast<MapJob*>(tables_ + o_maps);
+    d_bmaps_ = reinterpret_cast<MapJob*>(tables_ + o_bmaps);
     d_sjobs_ = reinterpret_cast<ScoreJob*>(tables_ + o_sjobs);
+    d_ccounts_ = reinterpret_cast<int*>(tables_ + o_ccounts);
+    d_cmaps_ = reinterpret_cast<MapJob*>(tables_ + o_cmaps);
+    d_cbmaps_ = reinterpret_cast<MapJob*>(tables_ + o_cbmaps);
+    d_csjobs_ = reinterpret_cast<ScoreJob*>(tables_ + o_csjobs);
 }
 
 Ssimu2Batch::~Ssimu2Batch()
@@ -99,6 +113,7 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
     maps_.clear();
     sjobs_.clear();
     xyb_off_.clear();
+    pair_jobs_.clear();
     exact_off_.assign(in.npairs, -1);
     const size_t elem = fp16_ ? 2 : 4;
     int64_t elems = 0;
@@ -170,12 +185,14 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
         j.tile_start = L.front_tiles_per_image;  // tile count until the jobs are ordered
         j.pair = i;
         (j.shared ? shared_jobs : own_jobs).push_back(j);
-        ScoreJob sj = {(int32_t)maps_.size(), (int32_t)L.maps.size(), {0, 0}};
+        ScoreJob sj = {(int32_t)maps_.size(), (int32_t)L.maps.size(), (int32_t)i, 0};
         sjobs_.push_back(sj);
+        pair_jobs_.push_back((int)maps_.size());
         for (const MapJob& m0 : L.maps) {
             MapJob m = m0;
             m.pair = i;
             m.tile_start = btiles + m0.tile_start;
+            m.part_start = m.tile_start;
             // a/b hold element offsets until the arena is known
             m.a = reinterpret_cast<const void*>(xyb_off_[2 * i] + reinterpret_cast<int64_t>(m0.a));
             m.b = reinterpret_cast<const void*>(xyb_off_[2 * i + 1] + reinterpret_cast<int64_t>(m0.b));
@@ -260,16 +277,31 @@ void Ssimu2Batch::ensure_capacity()
         const int64_t e = exact_off_[j.pair];
         j.exact0 = e < 0 ? nullptr : reinterpret_cast<float*>(static_cast<uint8_t*>(xyb_) + e * elem);
     }
-    std::vector<MapJob> dev_maps = maps_;
-    for (auto& m : dev_maps) {
+    dev_maps_ = maps_;
+    for (auto& m : dev_maps_) {
         m.a = static_cast<const uint8_t*>(xyb_) + reinterpret_cast<int64_t>(m.a) * elem;
         m.b = static_cast<const uint8_t*>(xyb_) + reinterpret_cast<int64_t>(m.b) * elem;
+    }
+    // the blur launch takes the jobs of the scales this call runs, numbered
+    // afresh (their partial-sum slots keep the numbering over every job); the
+    // score kernel takes every job, with no map flag on those that do not run
+    std::vector<MapJob> bmaps, smaps = dev_maps_;
+    int btotal = 0;
+    for (size_t k = 0; k < dev_maps_.size(); k++) {
+        if (call_mask_ & (1 << job_scale(dev_maps_[k]))) {
+            MapJob m = dev_maps_[k];
+            m.tile_start = btotal;
+            btotal += m.ntiles;
+            bmaps.push_back(m);
+        } else {
+            smaps[k].flags = 0;
+        }
     }
     int counts[kCountSlots] = {};
     counts[kCountFrontTiles] = front_tiles_;
     counts[kCountFrontJobs] = (int)fjobs_.size();
-    counts[kCountBlurTiles] = blur_tiles_;
-    counts[kCountBlurJobs] = (int)dev_maps.size();
+    counts[kCountBlurTiles] = btotal;
+    counts[kCountBlurJobs] = (int)bmaps.size();
     counts[kCountPairs] = table_pairs_;
     counts[kCountFrontShared] = front_shared_;
     auto put = [&](const void* dev, const void* src, size_t bytes) {
@@ -279,7 +311,8 @@ void Ssimu2Batch::ensure_capacity()
     put(d_fjobs_, fjobs_.data(), fjobs_.size() * sizeof(FrontJob));
     put(d_colors_, colors_.data(), colors_.size() * sizeof(FrontColor));
     put(d_geoms_, geoms_.data(), geoms_.size() * sizeof(PairGeom));
-    put(d_maps_, dev_maps.data(), dev_maps.size() * sizeof(MapJob));
+    put(d_maps_, smaps.data(), smaps.size() * sizeof(MapJob));
+    put(d_bmaps_, bmaps.data(), bmaps.size() * sizeof(MapJob));
     put(d_sjobs_, sjobs_.data(), sjobs_.size() * sizeof(ScoreJob));
     const size_t used = (reinterpret_cast<uint8_t*>(d_sjobs_) - tables_) + sjobs_.size() * sizeof(ScoreJob);
     VSHIP_HIP_CHECK(hipMemcpyAsync(tables_, h_tables_, used, hipMemcpyHostToDevice, stream_));
@@ -294,12 +327,12 @@ void Ssimu2Batch::launch_kernels(hipStream_t s)
         for (int phase = 0; phase < (front_phase1_ ? 2 : 1); phase++)
             VSHIP_HIP_CHECK(
                 launch_frontend<__half>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, front_spec_, phase, s));
-        VSHIP_HIP_CHECK(launch_blur<__half>(tuning_.blur_config, d_maps_, d_counts_, bgrid, taps_, partials_, s));
+        VSHIP_HIP_CHECK(launch_blur<__half>(tuning_.blur_config, d_bmaps_, d_counts_, bgrid, taps_, partials_, s));
     } else {
         for (int phase = 0; phase < (front_phase1_ ? 2 : 1); phase++)
             VSHIP_HIP_CHECK(
                 launch_frontend<float>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, front_spec_, phase, s));
-        VSHIP_HIP_CHECK(launch_blur<float>(tuning_.blur_config, d_maps_, d_counts_, bgrid, taps_, partials_, s));
+        VSHIP_HIP_CHECK(launch_blur<float>(tuning_.blur_config, d_bmaps_, d_counts_, bgrid, taps_, partials_, s));
     }
     VSHIP_HIP_CHECK(launch_score(d_sjobs_, d_counts_, kScoreGrid, d_maps_, partials_, d_features_, d_scores_, s));
 }
@@ -324,19 +357,86 @@ void Ssimu2Batch::capture()
     graph_valid_ = true;
 }
 
-void Ssimu2Batch::compute(const BatchInput& in, double* scores)
+void Ssimu2Batch::compute(const BatchInput& in, double* scores, int scale_mask)
 {
-    // features() and profile() refer to the last call that completed: none
-    // until this one has
+    // features(), profile() and complete() refer to the last call that
+    // completed: none until this one has
     npairs_ = 0;
+    if (scale_mask <= 0 || scale_mask > kAllScales)
+        VSHIP_THROW(Vship_BadPointer, "scale_mask must select scales among 0..4 (bits 0x1f)");
     VSHIP_HIP_CHECK(hipSetDevice(device_));
+    call_mask_ = scale_mask;
     build_tables(in);
     ensure_capacity();
     if (!graph_valid_) capture();
     VSHIP_HIP_CHECK(hipGraphLaunch(exec_, stream_));
     VSHIP_HIP_CHECK(hipStreamSynchronize(stream_));
     npairs_ = table_pairs_;
+    last_mask_ = scale_mask;
     std::memcpy(scores, h_scores_, (size_t)npairs_ * sizeof(double));
+}
+
+void Ssimu2Batch::complete(int count, const int* pairs, int scale_mask, double* scores)
+{
+    if (npairs_ == 0) VSHIP_THROW(Vship_BadPointer, "completion needs one successful compute call first");
+    if (scale_mask <= 0 || scale_mask > kAllScales)
+        VSHIP_THROW(Vship_BadPointer, "scale_mask must select scales among 0..4 (bits 0x1f)");
+    if ((scale_mask | last_mask_) != kAllScales)
+        VSHIP_THROW(Vship_BadPointer, "the last compute's scales and scale_mask must together cover scales 0..4");
+    if (count <= 0 || count > npairs_ || !pairs || !scores)
+        VSHIP_THROW(Vship_BadPointer, "count outside [1, pairs of the last compute] or null array");
+    VSHIP_HIP_CHECK(hipSetDevice(device_));
+    std::vector<char> seen((size_t)npairs_, 0);
+    std::vector<MapJob> bmaps, smaps;
+    std::vector<ScoreJob> sj;
+    int btotal = 0;
+    for (int c = 0; c < count; c++) {
+        const int i = pairs[c];
+        if (i < 0 || i >= npairs_) VSHIP_THROW(Vship_BadPointer, "pair index outside the last compute");
+        if (seen[(size_t)i]) VSHIP_THROW(Vship_BadPointer, "pair index listed twice");
+        seen[(size_t)i] = 1;
+        const int begin = pair_jobs_[(size_t)i], n = sjobs_[(size_t)i].job_count;
+        sj.push_back(ScoreJob{(int32_t)smaps.size(), n, i, 0});
+        for (int k = 0; k < n; k++) {
+            const MapJob& m = dev_maps_[(size_t)(begin + k)];
+            smaps.push_back(m);
+            if (scale_mask & (1 << job_scale(m))) {
+                MapJob b = m;
+                b.tile_start = btotal;
+                btotal += b.ntiles;
+                bmaps.push_back(b);
+            }
+        }
+    }
+    int counts[kCountSlots] = {};
+    counts[kCountBlurTiles] = btotal;
+    counts[kCountBlurJobs] = (int)bmaps.size();
+    counts[kCountPairs] = count;
+    auto put = [&](const void* dev, const void* src, size_t bytes) {
+        std::memcpy(h_tables_ + (static_cast<const uint8_t*>(dev) - tables_), src, bytes);
+    };
+    put(d_ccounts_, counts, sizeof(counts));
+    put(d_cmaps_, smaps.data(), smaps.size() * sizeof(MapJob));
+    put(d_cbmaps_, bmaps.data(), bmaps.size() * sizeof(MapJob));
+    put(d_csjobs_, sj.data(), sj.size() * sizeof(ScoreJob));
+    const size_t first = reinterpret_cast<uint8_t*>(d_ccounts_) - tables_;
+    const size_t last = (reinterpret_cast<uint8_t*>(d_csjobs_) - tables_) + sj.size() * sizeof(ScoreJob);
+    VSHIP_HIP_CHECK(hipMemcpyAsync(tables_ + first, h_tables_ + first, last - first, hipMemcpyHostToDevice, stream_));
+    const int bgrid = std::min((std::max(1, tuning_.blur_waves) * cus_ + 7) / 8 * 8, blur_grid_for(btotal));
+    if (btotal > 0) {
+        if (fp16_)
+            VSHIP_HIP_CHECK(launch_blur<__half>(tuning_.blur_config, d_cbmaps_, d_ccounts_, bgrid, taps_, partials_,
+                                                stream_));
+        else
+            VSHIP_HIP_CHECK(launch_blur<float>(tuning_.blur_config, d_cbmaps_, d_ccounts_, bgrid, taps_, partials_,
+                                               stream_));
+    }
+    VSHIP_HIP_CHECK(launch_score(d_csjobs_, d_ccounts_, std::min(count, kScoreGrid), d_cmaps_, partials_, d_features_,
+                                 d_scores_, stream_));
+    VSHIP_HIP_CHECK(hipMemcpyAsync(h_scores_, d_scores_, (size_t)npairs_ * sizeof(double), hipMemcpyDeviceToHost,
+                                   stream_));
+    VSHIP_HIP_CHECK(hipStreamSynchronize(stream_));
+    for (int c = 0; c < count; c++) scores[c] = h_scores_[pairs[c]];
 }
 
 void Ssimu2Batch::features(double* out)
@@ -358,6 +458,7 @@ void Ssimu2Batch::set_tuning(const BatchTuning& t)
 void Ssimu2Batch::autotune(const BatchInput& in, int reps, BatchTuning* chosen, double* best_ms)
 {
     npairs_ = 0;  // the replays leave features of no completed compute
+    call_mask_ = kAllScales;
     VSHIP_HIP_CHECK(hipSetDevice(device_));
     reps = std::max(1, reps);
     hipEvent_t e0, e1;
@@ -423,8 +524,8 @@ void Ssimu2Batch::profile(int iters, double* ms_front, double* ms_blur, double* 
                                                        front_spec_, phase, stream_));
         }
         VSHIP_HIP_CHECK(hipEventRecord(ev[1], stream_));
-        if (fp16_) VSHIP_HIP_CHECK(launch_blur<__half>(tuning_.blur_config, d_maps_, d_counts_, bgrid, taps_, partials_, stream_));
-        else VSHIP_HIP_CHECK(launch_blur<float>(tuning_.blur_config, d_maps_, d_counts_, bgrid, taps_, partials_, stream_));
+        if (fp16_) VSHIP_HIP_CHECK(launch_blur<__half>(tuning_.blur_config, d_bmaps_, d_counts_, bgrid, taps_, partials_, stream_));
+        else VSHIP_HIP_CHECK(launch_blur<float>(tuning_.blur_config, d_bmaps_, d_counts_, bgrid, taps_, partials_, stream_));
         VSHIP_HIP_CHECK(hipEventRecord(ev[2], stream_));
         VSHIP_HIP_CHECK(launch_score(d_sjobs_, d_counts_, kScoreGrid, d_maps_, partials_, d_features_, d_scores_, stream_));
         VSHIP_HIP_CHECK(hipEventRecord(ev[3], stream_));

@@ -44,6 +44,9 @@ struct BatchInput {
     const int64_t* ls2;
 };
 
+// Bit s selects scale s; the score uses scales 0-4.
+constexpr int kAllScales = (1 << kUsedScales) - 1;
+
 class Ssimu2Batch {
 public:
     Ssimu2Batch(int max_pairs, bool fp16, int device);
@@ -52,7 +55,16 @@ public:
     Ssimu2Batch& operator=(const Ssimu2Batch&) = delete;
 
     int device() const { return device_; }
-    void compute(const BatchInput& in, double* scores);
+    // scale_mask < kAllScales blurs only the map jobs of those scales (the
+    // planes of every scale are still stored): features of the other scales
+    // are 0 and the scores are those of the computed scales only.
+    void compute(const BatchInput& in, double* scores, int scale_mask = kAllScales);
+    // Blurs the map jobs of the scales in scale_mask for `count` pairs
+    // (indices into the last compute) on the planes that compute stored, and
+    // writes their full scores and features; the last compute's scales and
+    // scale_mask together must cover scales 0-4. Scores and features equal a
+    // full compute's bit for bit.
+    void complete(int count, const int* pairs, int scale_mask, double* scores);
     void features(double* out);            // npairs * 108 of the last compute
     void autotune(const BatchInput& in, int reps, BatchTuning* chosen, double* best_ms);
     void set_tuning(const BatchTuning& t);
@@ -79,7 +91,8 @@ private:
     std::vector<FrontJob> fjobs_;
     std::vector<FrontColor> colors_;
     std::vector<PairGeom> geoms_;
-    std::vector<MapJob> maps_;
+    std::vector<MapJob> maps_;       // every map job of the batch, a/b as arena offsets
+    std::vector<MapJob> dev_maps_;   // the same with device plane pointers
     std::vector<ScoreJob> sjobs_;
     std::vector<int64_t> xyb_off_;   // per image (2 per pair): element offset in the arena
     // per pair, FP16 layout: arena offset (FP16 elements) of the FP32 values of
@@ -96,6 +109,9 @@ private:
     int front_tiles_ = 0, blur_tiles_ = 0;
     int table_pairs_ = 0;  // pairs of the tables built last
     int npairs_ = 0;       // pairs of the last compute that completed, 0 if none
+    int last_mask_ = 0;    // scales blurred by that compute
+    int call_mask_ = kAllScales;  // scales the tables being built run
+    std::vector<int> pair_jobs_;  // per pair: first index into maps_ (and count in maps_ order)
     double work_bytes_ = 0.0, work_flops_ = 0.0;
 
     // device
@@ -109,13 +125,20 @@ private:
     double* d_features_ = nullptr;
     double* d_scores_ = nullptr;
     double* h_scores_ = nullptr;     // pinned
-    // table views inside tables_
+    // table views inside tables_: the blur launch reads the jobs of the
+    // scales it runs (d_bmaps_), the score kernel every job (d_maps_, flags
+    // cleared for jobs that did not run); the completion's own tables follow
     int* d_counts_ = nullptr;
     FrontJob* d_fjobs_ = nullptr;
     FrontColor* d_colors_ = nullptr;
     PairGeom* d_geoms_ = nullptr;
     MapJob* d_maps_ = nullptr;
+    MapJob* d_bmaps_ = nullptr;
     ScoreJob* d_sjobs_ = nullptr;
+    int* d_ccounts_ = nullptr;
+    MapJob* d_cmaps_ = nullptr;
+    MapJob* d_cbmaps_ = nullptr;
+    ScoreJob* d_csjobs_ = nullptr;
     int cap_fjobs_ = 0, cap_maps_ = 0;
 
     hipGraph_t graph_ = nullptr;

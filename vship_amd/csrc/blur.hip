@@ -270,9 +270,9 @@ void blur_map_kernel(const MapJob* __restrict__ jobs, const int* __restrict__ co
         const int tile_y = local / job.tiles_x;
         __syncthreads();  // LDS of the previous item is free
         if (job.flags & kMapSsim)
-            blur_tile<StoreT, TW, TH, 4>(job, tile_x, tile_y, taps, lds, red, partials + (int64_t)item * kPartialStride);
+            blur_tile<StoreT, TW, TH, 4>(job, tile_x, tile_y, taps, lds, red, partials + (int64_t)(job.part_start + local) * kPartialStride);
         else
-            blur_tile<StoreT, TW, TH, 2>(job, tile_x, tile_y, taps, lds, red, partials + (int64_t)item * kPartialStride);
+            blur_tile<StoreT, TW, TH, 2>(job, tile_x, tile_y, taps, lds, red, partials + (int64_t)(job.part_start + local) * kPartialStride);
     }
 }
 

@@ -383,10 +383,10 @@ void blur_mfma_kernel(const MapJob* __restrict__ jobs, const int* __restrict__ c
         __syncthreads();  // LDS of the previous item is free
         if (job.flags & kMapSsim)
             blur_tile_mfma<StoreT, 4>(job, tile_x, tile_y, taps, toe_hi, toe_lo, lds, red,
-                                      partials + (int64_t)item * kPartialStride);
+                                      partials + (int64_t)(job.part_start + local) * kPartialStride);
         else
             blur_tile_mfma<StoreT, 2>(job, tile_x, tile_y, taps, toe_hi, toe_lo, lds, red,
-                                      partials + (int64_t)item * kPartialStride);
+                                      partials + (int64_t)(job.part_start + local) * kPartialStride);
     }
 }
 

@@ -528,10 +528,10 @@ void blur_strip_kernel(const MapJob* __restrict__ jobs, const int* __restrict__ 
         __syncthreads();  // LDS of the previous item is free
         if (job.flags & kMapSsim)
             blur_item<StoreT, 4>(job, tile_x, tile_y, segb, taps, &toe, sA, sB, red,
-                                 partials + (int64_t)item * kPartialStride);
+                                 partials + (int64_t)(job.part_start + local) * kPartialStride);
         else
             blur_item<StoreT, 2>(job, tile_x, tile_y, segb, taps, &toe, sA, sB, red,
-                                 partials + (int64_t)item * kPartialStride);
+                                 partials + (int64_t)(job.part_start + local) * kPartialStride);
     }
 }
 

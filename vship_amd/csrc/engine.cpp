@@ -358,6 +358,7 @@ PairLayout make_pair_layout(int w, int h, int blur_config)
             m.tiles_x = (m.w + bt.tw - 1) / bt.tw;
             m.ntiles = m.tiles_x * ((m.h + bt.th - 1) / bt.th);
             m.tile_start = tiles;
+            m.part_start = tiles;
             m.pair = 0;
             m.feature_base = p * 36 + s * 6;
             tiles += m.ntiles;
@@ -547,7 +548,7 @@ double score_from_partials(const MapJob* maps, int nmaps, const float* partials,
         const MapJob& job = maps[jn];
         double acc[6][64] = {};
         for (int k = 0; k < job.ntiles; k++) {
-            const float* p = partials + (size_t)(job.tile_start + k) * kPartialStride;
+            const float* p = partials + (size_t)(job.part_start + k) * kPartialStride;
             for (int m = 0; m < 6; m++) acc[m][k & 63] += (double)p[m];
         }
         const double n = (double)job.w * (double)job.h;

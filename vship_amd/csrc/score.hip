@@ -46,7 +46,7 @@ void score_kernel(const ScoreJob* __restrict__ pairs, const int* __restrict__ co
             const MapJob job = jobs[pj.job_begin + j];
             double acc[6] = {0, 0, 0, 0, 0, 0};
             for (int k = lane; k < job.ntiles; k += 64) {
-                const float* p = partials + (int64_t)(job.tile_start + k) * kPartialStride;
+                const float* p = partials + (int64_t)(job.part_start + k) * kPartialStride;
 #pragma unroll
                 for (int m = 0; m < 6; m++) acc[m] += (double)p[m];
             }
@@ -77,11 +77,11 @@ void score_kernel(const ScoreJob* __restrict__ pairs, const int* __restrict__ co
 #pragma clang fp contract(off)
                 s *= 0.9562382616834844;
                 s = (6.248496625763138e-5 * s * s) * s + 2.326765642916932 * s - 0.020884521182843837 * s * s;
-                scores[pi] = (s > 0.0) ? pow(s, 0.6276336467831387) * -10.0 + 100.0 : 100.0;
+                scores[pj.pair] = (s > 0.0) ? pow(s, 0.6276336467831387) * -10.0 + 100.0 : 100.0;
             }
         }
         if (features)
-            for (int i = t; i < 108; i += kScoreThreads) features[(int64_t)pi * 108 + i] = feat[i];
+            for (int i = t; i < 108; i += kScoreThreads) features[(int64_t)pj.pair * 108 + i] = feat[i];
         __syncthreads();
     }
 }

@@ -146,11 +146,12 @@ struct MapJob {
     int32_t w, h, pitch;
     int32_t flags;             // MapFlags
     int32_t tiles_x;
-    int32_t tile_start;        // first global tile index == first partial slot
+    int32_t tile_start;        // first work-item index of the job in its launch
     int32_t ntiles;
     int32_t pair;
     int32_t feature_base;      // plane*36 + scale*6
-    int32_t pad;
+    int32_t part_start;        // partial-sum slot of the job's first item (== tile_start
+                               // unless a launch covers a subset of the jobs)
 };
 
 // Blur taps (float, like the reference's GaussianHandle, gaussianblur.hpp:7-14)
@@ -165,7 +166,8 @@ struct BlurTaps {
 struct ScoreJob {
     int32_t job_begin;         // first MapJob of the pair
     int32_t job_count;
-    int32_t pad[2];
+    int32_t pair;              // slot of its score and features
+    int32_t pad;
 };
 
 }  // namespace vship

@@ -169,6 +169,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "VshipAMD_SSIMU2BatchFree": (E, [Handler]),
         "VshipAMD_SSIMU2BatchCompute": (E, [Handler, ctypes.c_int, P(Colorspace), P(Colorspace), u8pp, u8pp, i64p,
                                             i64p, P(ctypes.c_double)]),
+        "VshipAMD_SSIMU2BatchComputeScales": (E, [Handler, ctypes.c_int, P(Colorspace), P(Colorspace), u8pp, u8pp,
+                                                  i64p, i64p, ctypes.c_int, P(ctypes.c_double)]),
+        "VshipAMD_SSIMU2BatchCompleteScales": (E, [Handler, ctypes.c_int, P(ctypes.c_int), ctypes.c_int,
+                                                   P(ctypes.c_double)]),
         "VshipAMD_SSIMU2BatchAutotune": (E, [Handler, ctypes.c_int, P(Colorspace), P(Colorspace), u8pp, u8pp, i64p,
                                              i64p, ctypes.c_int, P(BatchTuning), P(ctypes.c_double)]),
         "VshipAMD_SSIMU2BatchSetTuning": (E, [Handler, BatchTuning]),
@@ -372,6 +376,7 @@ def compute_device_noalloc(src: Colorspace, dis: Colorspace, dptrs1: Sequence[in
 
 
 STORAGE_FP32, STORAGE_FP16 = 0, 1
+ALL_SCALES = 0x1F  # scale_mask bit s = pyramid scale s; the score uses scales 0-4
 
 
 class SSIMU2Batch:
@@ -408,11 +413,27 @@ class SSIMU2Batch:
         C caller holds its descriptor and pointer arrays); see compute_prepared."""
         return self._args(src, dis, planes1, planes2, strides1, strides2)
 
-    def compute_prepared(self, args, out: np.ndarray) -> np.ndarray:
-        """VshipAMD_SSIMU2BatchCompute on arguments from prepare; scores into out (float64, >= n)."""
+    def compute_prepared(self, args, out: np.ndarray, scale_mask: int = ALL_SCALES) -> np.ndarray:
+        """VshipAMD_SSIMU2BatchCompute (or, with scale_mask != ALL_SCALES, ComputeScales) on arguments from
+        prepare; scores into out (float64, >= n)."""
         n, cs1, cs2, p1, p2, l1, l2 = args
-        check(self._lib.VshipAMD_SSIMU2BatchCompute(self.handle, n, cs1, cs2, p1, p2, l1, l2,
-                                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        dp = out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        if scale_mask == ALL_SCALES:
+            check(self._lib.VshipAMD_SSIMU2BatchCompute(self.handle, n, cs1, cs2, p1, p2, l1, l2, dp))
+        else:
+            check(self._lib.VshipAMD_SSIMU2BatchComputeScales(self.handle, n, cs1, cs2, p1, p2, l1, l2, scale_mask,
+                                                               dp))
+        return out
+
+    def complete_scales(self, pairs, scale_mask: int) -> np.ndarray:
+        """VshipAMD_SSIMU2BatchCompleteScales: full scores of pairs (indices into the last compute) after
+        blurring the scales in scale_mask on the planes that compute stored."""
+        idx = np.ascontiguousarray(pairs, dtype=np.int32)
+        out = np.zeros(len(idx), dtype=np.float64)
+        check(self._lib.VshipAMD_SSIMU2BatchCompleteScales(self.handle, len(idx),
+                                                            idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                                            scale_mask,
+                                                            out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
 
     def autotune(self, src, dis, planes1, planes2, strides1, strides2, reps: int = 2):

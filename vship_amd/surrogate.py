@@ -426,6 +426,12 @@ def screening_set(sources: int = 64, candidates: int = 16, width: int = 1920, he
     return refs, dists, qs
 
 
+# scale masks of the two screening passes: the network's inputs are the
+# scale 0-2 features; confirming a candidate exactly adds scales 3-4
+FEATURE_SCALES = 0b00111
+REST_SCALES = S.ALL_SCALES & ~FEATURE_SCALES
+
+
 def _select(scores, qs, target):
     """Highest-q candidate scoring at least `target`; the best-scoring one
     if none does."""
@@ -465,11 +471,13 @@ class Screening:
         self.pixels = width * height
         self._prepared = {}  # call arguments of the full candidate list
 
-    def _run(self, batch, items):
-        """Exact scores and features of (source, candidate) items in one
-        batched call. The source planes are passed once per source (the same
-        device pointers), so the batch computes each source's pyramid once;
-        the call arguments of the full candidate list are built once."""
+    def _run(self, batch, items, scale_mask=S.ALL_SCALES, features=True):
+        """Scores and features of (source, candidate) items in one batched
+        call (scale_mask < ALL_SCALES: only those scales blurred, see
+        SSIMU2Batch.complete_scales). The source planes are passed once per
+        source (the same device pointers), so the batch computes each source's
+        pyramid once; the call arguments of the full candidate list are built
+        once."""
         key = tuple(items)
         args = self._prepared.get(key)
         if args is None:
@@ -480,27 +488,32 @@ class Screening:
             args = batch.prepare(css, css, p1, p2, st, st)
             if len(items) == self.nsrc * self.ncand:
                 self._prepared[key] = args
-        scores = batch.compute_prepared(args, np.zeros(len(items), dtype=np.float64))
-        return scores, batch.features(len(items))
+        scores = batch.compute_prepared(args, np.zeros(len(items), dtype=np.float64), scale_mask)
+        return scores, (batch.features(len(items)) if features else None)
 
     def exact_only(self, batch, target):
         items = [(i, k) for i in range(self.nsrc) for k in range(self.ncand)]
-        scores, _ = self._run(batch, items)
+        scores, _ = self._run(batch, items, features=False)
         sc = scores.reshape(self.nsrc, self.ncand)
         return [_select(sc[i], self.qs, target) for i in range(self.nsrc)], sc
 
     def with_surrogate(self, batch, net, target, top=4, fallback_sigma=1.5):
-        """Surrogate-screened selection. Returns (choices, stats): the
-        network ranks every candidate from its scale 0-2 features; per
-        source the `top` highest-q candidates predicted to reach the target
-        (margin: two predicted standard deviations) plus every candidate
-        whose predicted standard deviation exceeds fallback_sigma are scored
-        by the exact metric; a source whose shortlist has no candidate
-        reaching the target falls back to exact scores of all its
-        candidates."""
+        """Surrogate-screened selection. Returns (choices, stats, mean, sigma).
+
+        One batched call blurs scales 0-2 of every candidate (the network's
+        inputs; the planes of scales 3-4 are stored on the way). The network
+        ranks every candidate; per source the `top` highest-q candidates
+        predicted to reach the target (margin: two predicted standard
+        deviations) plus every candidate whose predicted standard deviation
+        exceeds fallback_sigma are confirmed by the exact metric, which only
+        has scales 3-4 left to blur for them (their exact scores equal a full
+        evaluation's bit for bit); a source whose shortlist has no candidate
+        reaching the target is completed for all its candidates and selected
+        exactly. stats["confirmed"] maps (source, candidate) to the exact
+        scores used."""
         items = [(i, k) for i in range(self.nsrc) for k in range(self.ncand)]
         t0 = time.perf_counter()
-        _, feats = self._run(batch, items)
+        _, feats = self._run(batch, items, FEATURE_SCALES)
         t1 = time.perf_counter()
         X = inputs_from_features(feats, np.full(len(items), float(self.pixels)))
         mean, sigma = net.predict(X)
@@ -518,10 +531,8 @@ class Screening:
             fallback_items += len(unsure)
             short.append(sorted(set(likely) | set(unsure)))
         confirm = [(i, k) for i in range(self.nsrc) for k in short[i]]
-        cscores, _ = self._run(batch, confirm)
-        exact = {}
-        for (i, k), v in zip(confirm, cscores):
-            exact[(i, k)] = v
+        cscores = batch.complete_scales([i * self.ncand + k for i, k in confirm], REST_SCALES)
+        exact = dict(zip(confirm, cscores))
         choices, full_fallback = [], []
         for i in range(self.nsrc):
             ok = [k for k in short[i] if exact[(i, k)] >= target]
@@ -533,17 +544,15 @@ class Screening:
         if full_fallback:
             rest = [(i, k) for i in full_fallback for k in range(self.ncand) if (i, k) not in exact]
             if rest:
-                rs, _ = self._run(batch, rest)
-                for (i, k), v in zip(rest, rs):
-                    exact[(i, k)] = v
+                rs = batch.complete_scales([i * self.ncand + k for i, k in rest], REST_SCALES)
+                exact.update(zip(rest, rs))
             for i in full_fallback:
                 choices[i] = _select([exact[(i, k)] for k in range(self.ncand)], self.qs, target)
         t3 = time.perf_counter()
         stats = {
             "feature_pass_s": t1 - t0, "network_s": t2 - t1, "confirm_s": t3 - t2, "total_s": t3 - t0,
             "confirmed_pairs": len(confirm), "uncertain_pairs_sent_to_exact": fallback_items,
-            "sources_fully_rescored": len(full_fallback),
-            "pred_abs_error_mean": None,
+            "sources_fully_rescored": len(full_fallback), "confirmed": exact,
         }
         return choices, stats, mean, sigma
 
