@@ -409,3 +409,24 @@ def test_two_step_scales_equal_a_full_compute(pairs, storage):
             with pytest.raises(S.VshipError) as e:
                 b.complete_scales(bad_idx, mask)
             assert e.value.code == 11
+
+
+@pytest.mark.parametrize("config", [6, 7, 8])
+def test_strip_blur_keeps_22_bit_operands(pairs, config):
+    """The streamed matrix-core blur splits every FP32 operand into FP16 hi +
+    lo (about 22 significant bits): on FP32 planes its 108 features match the
+    FP32 VALU blur (config 0) to rounding level. An 11-bit blur (hi parts
+    alone) would move the features by about 1e-3 relative."""
+    big = [i for i, (w, h) in enumerate(SIZES) if w >= 64 and h >= 64]
+    n = len(SIZES)
+    with S.SSIMU2Batch(16, S.STORAGE_FP32) as b:
+        b.set_tuning(0, 2, 2)
+        b.compute(*pairs.args())
+        want = b.features(n)
+        b.set_tuning(config, 2, 2)
+        b.compute(*pairs.args())
+        got = b.features(n)
+    w, g = want[big], got[big]
+    rel = np.abs(g - w) / np.maximum(np.abs(w), 1e-3)
+    _say(f"strip config {config}: max relative feature difference {rel.max():.2e}")
+    np.testing.assert_allclose(g, w, rtol=1e-4, atol=1e-7)

@@ -178,17 +178,25 @@ def test_line_sizes_beyond_32bit_sample_offsets():
     try:
         with S.SSIMU2Handler(cs, cs) as hd:
             compact = hd.compute_device(dptr[0], dptr[1], strides, strides)
+            compact_f = hd.features()
             wide = hd.compute_device(dptr_big[0], dptr_big[1], big, big)
+            wide_f = hd.features()
         with S.SSIMU2Batch(2, S.STORAGE_FP32) as b:
             b.set_tuning(7, 2, 2)
             both = b.compute([cs, cs], [cs, cs], [dptr[0], dptr_big[0]], [dptr[1], dptr_big[1]], [strides, big],
                              [strides, big])
+            both_f = b.features(2)
     finally:
         for b in bufs:
             b.free()
     _say("wide rows", compact, wide, both)
     assert wide == compact
-    assert both[0] == both[1] == compact
+    np.testing.assert_array_equal(wide_f, compact_f)
+    assert both[0] == both[1]
+    np.testing.assert_array_equal(both_f[0], compact_f)
+    np.testing.assert_array_equal(both_f[1], compact_f)
+    # same features; the handler's final pow runs on the host, the batch's on the GPU
+    assert abs(both[0] - compact) <= 1e-12
 
 
 def test_mixed_colourspaces_in_one_pair():

@@ -25,15 +25,14 @@
 // Precision: FP16 operands carry FP32 values as hi + lo (hi = fp16 of x
 // rounded toward zero, lo = fp16(x - hi)), the taps likewise (round to
 // nearest); each product is hi*hi + lo*hi + hi*lo with FP32 accumulation,
-// about 22 significant bits. Every wave scales each window block by a power of
-// two from the largest magnitude in its own 16x32 operand block, so hi and lo
-// stay in FP16's normal range; the vertical products of the two blocks that
-// meet in one output block go to separate accumulators and are unscaled
-// (exactly) before the one FP32 add that joins them.
+// about 22 significant bits. Each item first finds the largest centred
+// magnitude in its whole window and scales every sample by one power of two
+// from it, so hi and lo stay in FP16's range for all four quantities and the
+// two vertical products of an output block share one accumulator.
 //
-// Identity: the blurred quantities are a, b, a*b and (a*a + b*b) / 2. For
-// a == b the last two are the same float and every operation on a and b is
-// the same, so sigma11 + sigma22 = 2 * sigma12 bit for bit and the map is 0.
+// Identity: the blurred quantities are a, b, a*b and (a - b)^2. For a == b
+// every operation on a and b is the same and (a - b)^2 is 0, so the SSIM
+// ratio's numerator and denominator are the same float and the map is 0.
 #include "ssimu2_common.hpp"
 #include "color_device.hpp"
 #include "blur_maps.hpp"
@@ -57,6 +56,7 @@ constexpr int kSlotFloats = 16 * kRP;         // one image, one block
 constexpr int kVecPerRow = kWinC / 4;         // 4-sample vectors per window row (20)
 constexpr int kVecPerBlock = 16 * kVecPerRow; // per image and block (320)
 constexpr int kStageIters = (kVecPerBlock + kBlurThreads - 1) / kBlurThreads;
+constexpr int kMaxSegb = 16;                  // items of at most 16 row blocks (256 rows)
 static_assert(kBlurThreads == 256, "four waves, one 16-column strip each");
 
 // Wave reductions on the DPP network: row shifts 1/2/4/8 reduce each 16-lane
@@ -94,22 +94,24 @@ __device__ inline const __attribute__((address_space(1))) T* global_ptr(const vo
     return (const __attribute__((address_space(1))) T*)p;
 }
 
-// x = v * s (s a power of two: exact) as hi + lo FP16 for two values: hi
-// rounded toward zero by one packed conversion, lo = fp16(x - hi) from one
-// mixed-precision FMA each (x - hi is exact in FP32).
-__device__ inline void split2(float v0, float v1, float s, _Float16& h0, _Float16& h1, _Float16& l0, _Float16& l1)
+// lo = fp16(x - hi) of two values whose packed FP16 hi parts are hh: x - hi
+// is exact in FP32 (hi is x truncated to 11 bits) and rounded once to FP16.
+// one is 1.0 hidden from the optimiser: fma(x, one, -hi) is then selected as
+// one mixed-precision FMA per value (hi read as FP16 directly) plus one pack,
+// where x - hi would cost a conversion and a subtraction per value. The
+// sequence is plain compiler code on purpose, so that the compiler inserts
+// the wait states an MFMA needs before it reads a VALU result.
+__device__ inline uint32_t lo_pair(float x0, float x1, fp16x2 hh, float one)
 {
-    const float x0 = v0 * s, x1 = v1 * s;
-    const fp16x2 hh = __builtin_amdgcn_cvt_pkrtz(x0, x1);
-    h0 = (_Float16)hh[0];
-    h1 = (_Float16)hh[1];
-    l0 = (_Float16)(x0 - (float)hh[0]);
-    l1 = (_Float16)(x1 - (float)hh[1]);
+    typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+    half2v l;
+    l[0] = (_Float16)fmaf(x0, one, -(float)hh[0]);
+    l[1] = (_Float16)fmaf(x1, one, -(float)hh[1]);
+    return __builtin_bit_cast(uint32_t, l);
 }
 
-// Four accumulator values (already scaled) as hi + lo FP16, hi rounded
-// toward zero.
-__device__ inline void split4(const float (&x)[4], half4& hi, half4& lo)
+// Four accumulator values as hi + lo FP16, hi rounded toward zero.
+__device__ inline void split4(const float (&x)[4], float one, half4& hi, half4& lo)
 {
     const fp16x2 h01 = __builtin_amdgcn_cvt_pkrtz(x[0], x[1]);
     const fp16x2 h23 = __builtin_amdgcn_cvt_pkrtz(x[2], x[3]);
@@ -117,10 +119,13 @@ __device__ inline void split4(const float (&x)[4], half4& hi, half4& lo)
     hi[1] = (_Float16)h01[1];
     hi[2] = (_Float16)h23[0];
     hi[3] = (_Float16)h23[1];
-    lo[0] = (_Float16)(x[0] - (float)h01[0]);
-    lo[1] = (_Float16)(x[1] - (float)h01[1]);
-    lo[2] = (_Float16)(x[2] - (float)h23[0]);
-    lo[3] = (_Float16)(x[3] - (float)h23[1]);
+    typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+    const half2v l01 = __builtin_bit_cast(half2v, lo_pair(x[0], x[1], h01, one));
+    const half2v l23 = __builtin_bit_cast(half2v, lo_pair(x[2], x[3], h23, one));
+    lo[0] = l01[0];
+    lo[1] = l01[1];
+    lo[2] = l23[0];
+    lo[3] = l23[1];
 }
 
 __device__ inline void split_rn(float x, _Float16& hi, _Float16& lo)
@@ -284,13 +289,44 @@ __device__ inline void stage_load(const void* A, const void* B, const StripGeom&
     }
 }
 
-// Centred values of the staged block into its ring slot: a - c and b - c
-// inside the image, 0 outside (rows outside were loaded as zeros, columns at
-// or beyond w are masked here). FP16 planes hold the distorted image as its
-// difference from the reference: b = a + d.
+// Largest centred magnitude |x - c| (as bit patterns, non-negative floats
+// order like their bits) of the image samples in a staged block: the item's
+// scale is chosen from it before any value is split.
 template <typename StoreT>
-__device__ inline void stage_store(const StageRegs<StoreT>& r, const StripGeom& g, int kb, float c, float* sA,
-                                   float* sB)
+__device__ inline uint32_t stage_max_bits(const StageRegs<StoreT>& r, const StripGeom& g, int kb, float c)
+{
+    const int t = threadIdx.x;
+    uint32_t m = 0;
+#pragma unroll
+    for (int it = 0; it < kStageIters; it++) {
+        const int e = t + it * kBlurThreads;
+        if (e >= kVecPerBlock) continue;
+        const int row = e / kVecPerRow, v = e - row * kVecPerRow;
+        const int gy = g.ty0 - kHalo + 16 * kb + row, gx = g.tx0 - kHalo + 4 * v;
+        const bool inside = gy >= 0 && gy < g.h && gx >= 0;
+        float av[4], bv[4];
+        unpack4(r.a[it], av);
+        unpack4(r.b[it], bv);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const float da = av[i] - c;
+            // FP16 planes hold the distorted image as its difference from the reference
+            const float db = (sizeof(StoreT) == 2) ? da + bv[i] : bv[i] - c;
+            const uint32_t bits = max(__float_as_uint(da) & 0x7fffffffu, __float_as_uint(db) & 0x7fffffffu);
+            m = (inside && gx + i < g.w) ? max(m, bits) : m;
+        }
+    }
+    return m;
+}
+
+// Scaled centred values of the staged block into its ring slot:
+// A = (a - c) s and B = (b - c) s inside the image, 0 outside (rows outside
+// were loaded as zeros, columns at or beyond w are masked here). s is a power
+// of two, so A = s * fl(a - c) exactly (one fma: a s - c s rounded once).
+// FP16 planes hold the distorted image as d = b - a: B = A + d s.
+template <typename StoreT>
+__device__ inline void stage_store(const StageRegs<StoreT>& r, const StripGeom& g, int kb, float s, float cs,
+                                   float* sA, float* sB)
 {
     const int t = threadIdx.x;
 #pragma unroll
@@ -307,9 +343,10 @@ __device__ inline void stage_store(const StageRegs<StoreT>& r, const StripGeom& 
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const bool in = inside && gx + i < g.w;
-            const float bb = (sizeof(StoreT) == 2) ? av[i] + bv[i] : bv[i];
-            va[i] = in ? av[i] - c : 0.0f;
-            vb[i] = in ? bb - c : 0.0f;
+            const float A = fmaf(av[i], s, -cs);
+            const float B = (sizeof(StoreT) == 2) ? fmaf(bv[i], s, A) : fmaf(bv[i], s, -cs);
+            va[i] = in ? A : 0.0f;
+            vb[i] = in ? B : 0.0f;
         }
         const int off = (kb % kSlots) * kSlotFloats + row * kRP + 4 * v;
         *reinterpret_cast<float4*>(sA + off) = make_float4(va[0], va[1], va[2], va[3]);
@@ -317,9 +354,49 @@ __device__ inline void stage_store(const StageRegs<StoreT>& r, const StripGeom& 
     }
 }
 
+// Accumulates the three hi/lo products of one 16x16x16 step onto c.
+__device__ inline floatx4 mma16_acc(const half4& ah, const half4& al, const half4& bh, const half4& bl, floatx4 c)
+{
+    c = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl, c, 0, 0, 0);
+    return c;
+}
+
+// Two values as hi + lo FP16 (hi rounded toward zero, lo = fp16(x - hi)).
+__device__ inline void split_pair(float x0, float x1, float one, _Float16& h0, _Float16& h1, _Float16& l0,
+                                  _Float16& l1)
+{
+    typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+    const fp16x2 hh = __builtin_amdgcn_cvt_pkrtz(x0, x1);
+    h0 = (_Float16)hh[0];
+    h1 = (_Float16)hh[1];
+    const half2v ll = __builtin_bit_cast(half2v, lo_pair(x0, x1, hh, one));
+    l0 = ll[0];
+    l1 = ll[1];
+}
+
+// One item. Every quantity is blurred in a scaled domain: the window values
+// are A = (a - c) s, B = (b - c) s with one power of two s for the whole item,
+// chosen so that |A|, |B| < 2^6 (so |A B| < 2^12 and (A - B)^2 < 2^14 stay in
+// FP16's range and hi/lo keep 22 significant bits; the subnormal floor of lo
+// is 2^-24 / s^2 of the item's magnitude, far below anything the maps
+// resolve). With one scale the two vertical products of an output block
+// accumulate in one MFMA chain. The blurred quantities are A, B, A B and
+// (A - B)^2, and sigma1^2 + sigma2^2 = 2 sigma12 + (E[(A - B)^2] - (mu1 - mu2)^2):
+// for a == b the last term is 0 exactly, so the SSIM ratio is 1 bit for bit.
+// The maps are evaluated on the scaled
+// quantities: the SSIM ratio's numerator and denominator both carry s^2
+// exactly (C2 becomes C2 s^2, (mu1 - mu2)^2 is unscaled by s^-2) and the edge
+// ratio (|B - M2| + s) / (|A - M1| + s) equals (|b - mu2| + 1) / (|a - mu1| + 1),
+// so the maps are those of the unscaled values bit for bit. Outputs outside
+// the image have zero normalisers and zero samples, which makes both maps 0
+// exactly (the SSIM ratio is C2 s^2 / C2 s^2 = 1, the edge ratio s / s), so
+// no output is masked by a branch.
 template <typename StoreT, int NQ>
 __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, const BlurTaps& taps,
-                          const ToeplitzLds* TL, float* sA, float* sB, float* red, float* __restrict__ partial)
+                          const ToeplitzLds* TL, float* sA, float* sB, float* red, float* rowinv,
+                          float* __restrict__ partial)
 {
     const int t = threadIdx.x;
     const int wave = t >> 6, lane = t & 63;
@@ -338,18 +415,46 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
     const int nrb = min(segb, (g.h - g.ty0 + 15) / 16);
     const int nwb = nrb + 1;
 
+    // 1 / (in-bounds tap sum) of each output row of the item, 0 below the image
+    if (t < 16 * segb) rowinv[t] = inv_norm(g.ty0 + t, g.h, taps);
+
+    // the item's scale: largest centred magnitude over the whole window (the
+    // blocks in reverse order, so block 0 is the one left in the registers)
     StageRegs<StoreT> st;
-    stage_load<StoreT>(job.a, job.b, g, 0, st);
-    stage_store<StoreT>(st, g, 0, c, sA, sB);
+    uint32_t mbits = 0;
+    for (int kb = nwb - 1; kb >= 0; kb--) {
+        stage_load<StoreT>(job.a, job.b, g, kb, st);
+        mbits = max(mbits, stage_max_bits<StoreT>(st, g, kb, c));
+    }
+    mbits = wave_max_bits(mbits);
+    uint32_t* redu = reinterpret_cast<uint32_t*>(red);
+    if (lane == 0) redu[wave] = mbits;
+    __syncthreads();
+    const float vmax = __uint_as_float(max(max(redu[0], redu[1]), max(redu[2], redu[3])));
+    int e1 = 6;
+    if (vmax > 0.0f) {
+        int ex;
+        frexpf(vmax, &ex);  // vmax < 2^ex
+        e1 = 6 - ex;
+    }
+    e1 = max(-60, min(e1, 60));
+    const float s = ldexpf(1.0f, e1);
+    const float cs = c * s;
+    const float inv_s2 = ldexpf(1.0f, -2 * e1);
+    const float c2s2 = kC2 * ldexpf(1.0f, 2 * e1);
+
+    stage_store<StoreT>(st, g, 0, s, cs, sA, sB);
     __syncthreads();
 
     const int x = g.tx0 + 16 * wave + ln;
-    const float inx = inv_norm(x, g.w, taps);
+    const float inx = inv_norm(x, g.w, taps);  // 0 at or beyond w
+    float one = 1.0f;
+    asm volatile("" : "+s"(one));  // emits nothing: keeps fma(x, one, -hi) an FMA
     const bool do_ssim = (job.flags & kMapSsim) != 0;
-    const bool do_edge = (job.flags & (kMapArtifact | kMapDetail)) != 0;
+    const bool do_art = (job.flags & kMapArtifact) != 0, do_det = (job.flags & kMapDetail) != 0;
+    const bool do_edge = do_art || do_det;
     float s1m[3] = {0.f, 0.f, 0.f}, s4m[3] = {0.f, 0.f, 0.f};
-    floatx4 prev[NQ];  // T0 part of the next output block (scale of its window block)
-    float prev_u1 = 0.f, prev_u2 = 0.f;
+    floatx4 acc[NQ];  // vertical accumulation of the output block in progress
 
     for (int kb = 0; kb < nwb; kb++) {
         const bool more = kb + 1 < nwb;
@@ -365,26 +470,9 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
             a[0] = a0.x; a[1] = a0.y; a[2] = a0.z; a[3] = a0.w; a[4] = a1.x; a[5] = a1.y; a[6] = a1.z; a[7] = a1.w;
             b[0] = b0.x; b[1] = b0.y; b[2] = b0.z; b[3] = b0.w; b[4] = b1.x; b[5] = b1.y; b[6] = b1.z; b[7] = b1.w;
         }
-        uint32_t mbits = 0;
-#pragma unroll
-        for (int j = 0; j < 8; j++)
-            mbits = max(mbits, max(__float_as_uint(a[j]) & 0x7fffffffu, __float_as_uint(b[j]) & 0x7fffffffu));
-        const float vmax = __uint_as_float(wave_max_bits(mbits));
-        // power-of-two scales: values to below 2^14, products to below 2^14
-        int e1 = 14, e2 = 14;
-        if (vmax > 0.0f) {
-            int ex;
-            frexpf(vmax, &ex);  // vmax < 2^ex
-            e1 = 14 - ex;
-            e2 = 14 - 2 * ex;
-        }
-        e1 = max(-60, min(e1, 60));
-        e2 = max(-60, min(e2, 60));
-        const float s1 = ldexpf(1.0f, e1), s2 = ldexpf(1.0f, e2), s2h = ldexpf(1.0f, e2 - 1);
-        const float u1 = ldexpf(1.0f, -e1), u2 = ldexpf(1.0f, -e2);
 
-        // ---- per quantity: operand hi/lo, horizontal product, hi/lo of the
-        // results (the vertical B operand)
+        // ---- per quantity (A, B, A B, (A - B)^2): hi/lo operand, horizontal
+        // product, hi/lo of the results (the vertical B operand)
         half4 bh[NQ], bl[NQ];
         const half8 g_hi = TL->g_hi[lane], g_lo = TL->g_lo[lane];
 #pragma unroll
@@ -399,35 +487,28 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
 #pragma clang fp contract(off)
                     if (q == 2) { v0 = a[j] * b[j]; v1 = a[j + 1] * b[j + 1]; }
                     else {
-                        // (a^2 + b^2) / 2: the halving is folded into the
-                        // power-of-two scale (exact either way)
-                        v0 = a[j] * a[j] + b[j] * b[j];
-                        v1 = a[j + 1] * a[j + 1] + b[j + 1] * b[j + 1];
+                        const float d0 = a[j] - b[j], d1 = a[j + 1] - b[j + 1];
+                        v0 = d0 * d0;
+                        v1 = d1 * d1;
                     }
                 }
                 _Float16 h0, h1, l0, l1;
-                split2(v0, v1, q < 2 ? s1 : (q == 2 ? s2 : s2h), h0, h1, l0, l1);
+                split_pair(v0, v1, one, h0, h1, l0, l1);
                 ah[j] = h0; ah[j + 1] = h1; al[j] = l0; al[j + 1] = l1;
             }
             const floatx4 hq = mma32(ah, al, g_hi, g_lo);
             const float hv[4] = {hq[0], hq[1], hq[2], hq[3]};
-            split4(hv, bh[q], bl[q]);
+            split4(hv, one, bh[q], bl[q]);
         }
 
-        // ---- output block kb - 1 is complete: T1 part, unscale, maps
+        // ---- output block kb - 1 is complete: T1 part, maps
         if (kb >= 1) {
             const int rbk = kb - 1;
             half4 t1_hi, t1_lo;
             split_half8(TL->t1[lane], t1_hi, t1_lo);
-            floatx4 v[NQ];
 #pragma unroll
-            for (int q = 0; q < NQ; q++) {
-                const floatx4 t1 = mma16(t1_hi, t1_lo, bh[q], bl[q]);
-                const float up = q < 2 ? prev_u1 : prev_u2, uc = q < 2 ? u1 : u2;
-#pragma unroll
-                for (int i = 0; i < 4; i++) v[q][i] = prev[q][i] * up + t1[i] * uc;
-            }
-            // centred samples at the outputs: window rows 16rbk + 8 + 4lg + i
+            for (int q = 0; q < NQ; q++) acc[q] = mma16_acc(t1_hi, t1_lo, bh[q], bl[q], acc[q]);
+            // centred scaled samples at the outputs: window rows 16rbk + 8 + 4lg + i
             const float* ca;
             const float* cb;
             {
@@ -437,42 +518,52 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
                 ca = sA + slot * kSlotFloats + rr * kRP + 16 * wave + kHalo + ln;
                 cb = sB + slot * kSlotFloats + rr * kRP + 16 * wave + kHalo + ln;
             }
+            const float4 rn = *reinterpret_cast<const float4*>(rowinv + 16 * rbk + 4 * lg);
+            const float rnv[4] = {rn.x, rn.y, rn.z, rn.w};
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                const int y = g.ty0 + 16 * rbk + 4 * lg + i;
-                if (x < g.w && y < g.h) {
-                    const float nrm = inx * inv_norm(y, g.h, taps);
-                    const float m1 = v[0][i] * nrm, m2 = v[1][i] * nrm;
-                    if constexpr (NQ == 4) {
-                        if (do_ssim) {
-                            const float s12 = v[2][i] * nrm;
-                            const float sq = (v[3][i] * nrm) * 2.0f;
-                            const float val = ssim_value(m1, m2, s12, sq);
-                            s1m[0] += val;
-                            s4m[0] += pow4(val);
-                        }
+                const float nrm = inx * rnv[i];
+                const float m1 = acc[0][i] * nrm, m2 = acc[1][i] * nrm;
+                if constexpr (NQ == 4) {
+                    if (do_ssim) {
+#pragma clang fp contract(off)
+                        const float s12 = acc[2][i] * nrm;
+                        const float dsq = acc[3][i] * nrm;
+                        const float md = m1 - m2, md2 = md * md;
+                        const float num_m = 1.0f - md2 * inv_s2;
+                        const float num_s = 2.0f * (s12 - m1 * m2) + c2s2;
+                        // sigma1^2 + sigma2^2 = 2 sigma12 + var(A - B)
+                        const float den_s = num_s + (dsq - md2);
+                        const float val = fmaxf(1.0f - div_nr(num_m * num_s, den_s), 0.0f);
+                        const float v2 = val * val;
+                        s1m[0] += val;
+                        s4m[0] = fmaf(v2, v2, s4m[0]);
                     }
-                    if (do_edge) {
-                        const float dd = edge_value(ca[i * kRP], cb[i * kRP], m1, m2);
-                        const float art = fmaxf(dd, 0.0f), det = fmaxf(-dd, 0.0f);
+                }
+                if (do_edge) {
+#pragma clang fp contract(off)
+                    const float dd = div_nr(fabsf(cb[i * kRP] - m2) + s, fabsf(ca[i * kRP] - m1) + s) - 1.0f;
+                    if (do_art) {
+                        const float art = fmaxf(dd, 0.0f), a2 = art * art;
                         s1m[1] += art;
-                        s4m[1] += pow4(art);
+                        s4m[1] = fmaf(a2, a2, s4m[1]);
+                    }
+                    if (do_det) {
+                        const float det = fmaxf(-dd, 0.0f), d2 = det * det;
                         s1m[2] += det;
-                        s4m[2] += pow4(det);
+                        s4m[2] = fmaf(d2, d2, s4m[2]);
                     }
                 }
             }
         }
-        // ---- T0 part of output block kb
+        // ---- T0 part of output block kb (starts its accumulation)
         if (kb < nrb) {
             half4 t0_hi, t0_lo;
             split_half8(TL->t01[lane], t0_hi, t0_lo);
 #pragma unroll
-            for (int q = 0; q < NQ; q++) prev[q] = mma16(t0_hi, t0_lo, bh[q], bl[q]);
-            prev_u1 = u1;
-            prev_u2 = u2;
+            for (int q = 0; q < NQ; q++) acc[q] = mma16(t0_hi, t0_lo, bh[q], bl[q]);
         }
-        if (more) stage_store<StoreT>(st, g, kb + 1, c, sA, sB);
+        if (more) stage_store<StoreT>(st, g, kb + 1, s, cs, sA, sB);
         __syncthreads();
     }
 
@@ -502,6 +593,7 @@ void blur_strip_kernel(const MapJob* __restrict__ jobs, const int* __restrict__ 
 {
     __shared__ __attribute__((aligned(16))) float ring[2 * kSlots * kSlotFloats];
     __shared__ float red[32];
+    __shared__ __attribute__((aligned(16))) float rowinv[16 * kMaxSegb];
     __shared__ ToeplitzLds toe;
     const int total = counts[kCountBlurTiles];
     const int njobs = counts[kCountBlurJobs];
@@ -527,10 +619,10 @@ void blur_strip_kernel(const MapJob* __restrict__ jobs, const int* __restrict__ 
         const int tile_y = local / job.tiles_x;
         __syncthreads();  // LDS of the previous item is free
         if (job.flags & kMapSsim)
-            blur_item<StoreT, 4>(job, tile_x, tile_y, segb, taps, &toe, sA, sB, red,
+            blur_item<StoreT, 4>(job, tile_x, tile_y, segb, taps, &toe, sA, sB, red, rowinv,
                                  partials + (int64_t)(job.part_start + local) * kPartialStride);
         else
-            blur_item<StoreT, 2>(job, tile_x, tile_y, segb, taps, &toe, sA, sB, red,
+            blur_item<StoreT, 2>(job, tile_x, tile_y, segb, taps, &toe, sA, sB, red, rowinv,
                                  partials + (int64_t)(job.part_start + local) * kPartialStride);
     }
 }
@@ -542,7 +634,7 @@ hipError_t launch_blur_strip(int segb, const MapJob* jobs, const int* counts, in
                              float* partials, hipStream_t stream)
 {
     if (grid <= 0) return hipSuccess;
-    if (grid % kXcds || segb < 1) return hipErrorInvalidValue;
+    if (grid % kXcds || segb < 1 || segb > kMaxSegb) return hipErrorInvalidValue;
     hipLaunchKernelGGL(blur_strip_kernel<StoreT>, dim3(grid), dim3(kBlurThreads), 0, stream, jobs, counts, taps, segb,
                        partials);
     return hipGetLastError();
