@@ -38,6 +38,8 @@
 #include "blur_maps.hpp"
 #include "kernels.hpp"
 
+#include <type_traits>
+
 namespace vship {
 
 namespace {
@@ -143,21 +145,23 @@ __device__ inline floatx4 mma32(const half8& ah, const half8& al, const half8& b
     return c;
 }
 
-__device__ inline floatx4 mma16(const half4& ah, const half4& al, const half4& bh, const half4& bl)
-{
-    floatx4 c = {0.f, 0.f, 0.f, 0.f};
-    c = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh, c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh, c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl, c, 0, 0, 0);
-    return c;
-}
-
 // Toeplitz fragments of one lane (select chains: a lane-varying index into
 // the taps would put them in scratch memory).
+//
+// Vertical pass: output block rb is Tfull * S with the 16x32 band
+// Tfull[m][r] = g[r - m] and S the 32 window rows of blocks rb and rb+1
+// (16 + 16). A 16x16x32 product contracts over k = 8*(lane/16) + j; the
+// horizontal results hold rows 4*(lane/16) + i of their block in each lane,
+// so k is mapped to window row r(k) = 4*(lane/16) + j for j < 4 (block rb)
+// and 16 + 4*(lane/16) + j - 4 for j >= 4 (block rb + 1). The B operand is
+// then the lane's four results of block rb followed by its four of block
+// rb + 1, straight from registers, and the A operand is Tfull with its
+// columns in that order (tv; tw holds the order with the two blocks'
+// halves swapped, for a register tuple whose halves alternate).
 struct Toeplitz {
     half8 g_hi, g_lo;    // horizontal B operand: k = 8*(lane/16)+j, n = lane%16, g[k - n]
-    half4 t0_hi, t0_lo;  // vertical A operand: m = lane%16, k = 4*(lane/16)+j, g[k - m]
-    half4 t1_hi, t1_lo;  //                                               g[16 + k - m]
+    half8 tv_hi, tv_lo;  // vertical A operand: m = lane%16, g[r(k) - m]
+    half8 tw_hi, tw_lo;  // the same with the two blocks' halves of k swapped
 };
 
 __device__ inline float tap_at(const BlurTaps& taps, int k)
@@ -178,53 +182,36 @@ __device__ inline Toeplitz make_toeplitz(const BlurTaps& taps, int lane)
         split_rn(tap_at(taps, 8 * lg + j - ln), h, l);
         T.g_hi[j] = h;
         T.g_lo[j] = l;
-    }
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        _Float16 h, l;
-        split_rn(tap_at(taps, 4 * lg + j - ln), h, l);
-        T.t0_hi[j] = h;
-        T.t0_lo[j] = l;
-        split_rn(tap_at(taps, 16 + 4 * lg + j - ln), h, l);
-        T.t1_hi[j] = h;
-        T.t1_lo[j] = l;
+        const int r = j < 4 ? 4 * lg + j : 16 + 4 * lg + (j - 4);
+        split_rn(tap_at(taps, r - ln), h, l);
+        T.tv_hi[j] = h;
+        T.tv_lo[j] = l;
+        const int rw = j < 4 ? 16 + 4 * lg + j : 4 * lg + (j - 4);
+        split_rn(tap_at(taps, rw - ln), h, l);
+        T.tw_hi[j] = h;
+        T.tw_lo[j] = l;
     }
     return T;
 }
 
 // The fragments of every lane, shared by the workgroup's waves through LDS
-// (structure of arrays, one 16-byte vector per lane and fragment pair, so the
+// (structure of arrays, one 16-byte vector per lane and fragment, so the
 // reads are conflict-free); loaded right before the products that use them,
 // which keeps 16 registers free across the item.
 struct ToeplitzLds {
     half8 g_hi[64], g_lo[64];
-    half8 t01[64];  // t0_hi, t0_lo
-    half8 t1[64];   // t1_hi, t1_lo
+    half8 tv_hi[64], tv_lo[64];
+    half8 tw_hi[64], tw_lo[64];
 };
 
 __device__ inline void toeplitz_to_lds(const Toeplitz& T, int lane, ToeplitzLds* L)
 {
     L->g_hi[lane] = T.g_hi;
     L->g_lo[lane] = T.g_lo;
-    half8 a, b;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        a[j] = T.t0_hi[j];
-        a[4 + j] = T.t0_lo[j];
-        b[j] = T.t1_hi[j];
-        b[4 + j] = T.t1_lo[j];
-    }
-    L->t01[lane] = a;
-    L->t1[lane] = b;
-}
-
-__device__ inline void split_half8(const half8& v, half4& lo4, half4& hi4)
-{
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        lo4[j] = v[j];
-        hi4[j] = v[4 + j];
-    }
+    L->tv_hi[lane] = T.tv_hi;
+    L->tv_lo[lane] = T.tv_lo;
+    L->tw_hi[lane] = T.tw_hi;
+    L->tw_lo[lane] = T.tw_lo;
 }
 
 template <typename StoreT>
@@ -238,7 +225,7 @@ struct StageRegs<float> {
 
 template <>
 struct StageRegs<__half> {
-    typedef uintx2 Vec;
+    typedef uintx2 Vec;  // four FP16 samples
     Vec a[kStageIters], b[kStageIters];
 };
 
@@ -255,6 +242,27 @@ __device__ inline void unpack4(const uintx2& q, float v[4])
     v[3] = __half2float(__ushort_as_half((unsigned short)(q.y >> 16)));
 }
 
+// Samples nv.. (nv in 1..3) of a vector that straddles the image's last
+// column are replaced by the fill values (the row padding is not defined).
+__device__ inline void fill_tail(floatx4& a, floatx4& b, int nv, const floatx4& fa, const floatx4& fb)
+{
+    a.w = fa.w;
+    b.w = fb.w;
+    if (nv < 3) { a.z = fa.z; b.z = fb.z; }
+    if (nv < 2) { a.y = fa.y; b.y = fb.y; }
+}
+
+__device__ inline void fill_tail(uintx2& a, uintx2& b, int nv, const uintx2& fa, const uintx2& fb)
+{
+    // samples 0 1 2 3 = low and high halves of x, then of y
+    a.y = nv < 3 ? fa.y : ((a.y & 0xffffu) | (fa.y & 0xffff0000u));
+    b.y = nv < 3 ? fb.y : ((b.y & 0xffffu) | (fb.y & 0xffff0000u));
+    if (nv < 2) {
+        a.x = (a.x & 0xffffu) | (fa.x & 0xffff0000u);
+        b.x = (b.x & 0xffffu) | (fb.x & 0xffff0000u);
+    }
+}
+
 // Window geometry of one item.
 struct StripGeom {
     int w, h, pitch;
@@ -263,9 +271,15 @@ struct StripGeom {
 
 // Global loads of window block kb (rows ty0 - 8 + 16kb ..) into registers.
 // Columns tx0 - 8 + 4v..+3 are a multiple of 4 and plane rows are padded to
-// 64 elements, so every vector lies entirely inside or outside its row.
+// 64 elements, so every vector that starts inside a row lies inside its
+// padded row. Samples outside the image take the fill values fa, fb: the
+// item's centring sample c (and, for FP16 planes, whose second image is
+// stored as its difference from the first, a zero difference), which the
+// staging maps to exactly 0, the value the renormalised blur gives samples
+// outside the image.
 template <typename StoreT>
-__device__ inline void stage_load(const void* A, const void* B, const StripGeom& g, int kb, StageRegs<StoreT>& r)
+__device__ inline void stage_load(const void* A, const void* B, const StripGeom& g, int kb, StageRegs<StoreT>& r,
+                                  const typename StageRegs<StoreT>::Vec& fa, const typename StageRegs<StoreT>::Vec& fb)
 {
     typedef typename StageRegs<StoreT>::Vec Vec;
     const auto* ga = global_ptr<Vec>(A);
@@ -276,57 +290,76 @@ __device__ inline void stage_load(const void* A, const void* B, const StripGeom&
         const int e = t + it * kBlurThreads;
         const int row = e / kVecPerRow, v = e - row * kVecPerRow;
         const int gy = g.ty0 - kHalo + 16 * kb + row, gx = g.tx0 - kHalo + 4 * v;
-        const bool ok = e < kVecPerBlock && gy >= 0 && gy < g.h && gx >= 0 && gx < g.pitch;
-        Vec va = {}, vb = {};
+        const bool ok = e < kVecPerBlock && gy >= 0 && gy < g.h && gx >= 0 && gx < g.w;
+        Vec va = fa, vb = fb;
         if (ok) {
             // vector index inside the plane (< 2^31 elements per plane)
             const uint32_t o = ((uint32_t)gy * (uint32_t)g.pitch + (uint32_t)gx) >> 2;
             va = ga[o];
             vb = gb[o];
+            if (gx + 4 > g.w) fill_tail(va, vb, g.w - gx, fa, fb);
         }
         r.a[it] = va;
         r.b[it] = vb;
     }
 }
 
-// Largest centred magnitude |x - c| (as bit patterns, non-negative floats
-// order like their bits) of the image samples in a staged block: the item's
-// scale is chosen from it before any value is split.
+// Bound on the centred magnitudes |x - c| of a staged window (fill samples
+// are c: they add nothing). Any bound within a small factor of the maximum
+// serves: it only picks the item's power-of-two scale.
 template <typename StoreT>
-__device__ inline uint32_t stage_max_bits(const StageRegs<StoreT>& r, const StripGeom& g, int kb, float c)
-{
-    const int t = threadIdx.x;
-    uint32_t m = 0;
+struct WindowBound;
+
+template <>
+struct WindowBound<float> {
+    float m = 0.0f;
+    __device__ inline explicit WindowBound(float) {}
+    __device__ inline void add(const StageRegs<float>& r, float c)
+    {
 #pragma unroll
-    for (int it = 0; it < kStageIters; it++) {
-        const int e = t + it * kBlurThreads;
-        if (e >= kVecPerBlock) continue;
-        const int row = e / kVecPerRow, v = e - row * kVecPerRow;
-        const int gy = g.ty0 - kHalo + 16 * kb + row, gx = g.tx0 - kHalo + 4 * v;
-        const bool inside = gy >= 0 && gy < g.h && gx >= 0;
-        float av[4], bv[4];
-        unpack4(r.a[it], av);
-        unpack4(r.b[it], bv);
+        for (int it = 0; it < kStageIters; it++) {
+            float av[4], bv[4];
+            unpack4(r.a[it], av);
+            unpack4(r.b[it], bv);
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const float da = av[i] - c;
-            // FP16 planes hold the distorted image as its difference from the reference
-            const float db = (sizeof(StoreT) == 2) ? da + bv[i] : bv[i] - c;
-            const uint32_t bits = max(__float_as_uint(da) & 0x7fffffffu, __float_as_uint(db) & 0x7fffffffu);
-            m = (inside && gx + i < g.w) ? max(m, bits) : m;
+            for (int i = 0; i < 4; i++) m = fmaxf(m, fmaxf(fabsf(av[i] - c), fabsf(bv[i] - c)));
         }
     }
-    return m;
-}
+    __device__ inline float value() const { return m; }
+};
 
-// Scaled centred values of the staged block into its ring slot:
-// A = (a - c) s and B = (b - c) s inside the image, 0 outside (rows outside
-// were loaded as zeros, columns at or beyond w are masked here). s is a power
-// of two, so A = s * fl(a - c) exactly (one fma: a s - c s rounded once).
-// FP16 planes hold the distorted image as d = b - a: B = A + d s.
+template <>
+struct WindowBound<__half> {
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    h2 mx = {0, 0}, mn = {0, 0};  // of a - c and of a + d - c, two samples per lane
+    h2 c2;
+    __device__ inline explicit WindowBound(float c) { c2 = h2{(_Float16)c, (_Float16)c}; }
+    __device__ inline void add(const StageRegs<__half>& r, float)
+    {
+#pragma unroll
+        for (int it = 0; it < kStageIters; it++) {
+            const uint32_t aw[2] = {r.a[it].x, r.a[it].y}, dw[2] = {r.b[it].x, r.b[it].y};
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const h2 da = __builtin_bit_cast(h2, aw[k]) - c2;  // packed FP16: a bound needs no more
+                const h2 db = da + __builtin_bit_cast(h2, dw[k]);
+                mx = __builtin_elementwise_max(mx, __builtin_elementwise_max(da, db));
+                mn = __builtin_elementwise_min(mn, __builtin_elementwise_min(da, db));
+            }
+        }
+    }
+    __device__ inline float value() const
+    {
+        return fmaxf(fmaxf((float)mx[0], (float)mx[1]), -fminf((float)mn[0], (float)mn[1]));
+    }
+};
+
+// Scaled centred values of a staged block into its ring slot: A = (a - c) s
+// and B = (b - c) s (0 for fill samples). s is a power of two, so
+// A = fma(a, s, -c s) is s * (a - c) rounded once. FP16 planes hold the
+// second image as d = b - a: B = A + d s.
 template <typename StoreT>
-__device__ inline void stage_store(const StageRegs<StoreT>& r, const StripGeom& g, int kb, float s, float cs,
-                                   float* sA, float* sB)
+__device__ inline void stage_store(const StageRegs<StoreT>& r, int kb, float s, float cs, float* sA, float* sB)
 {
     const int t = threadIdx.x;
 #pragma unroll
@@ -334,33 +367,19 @@ __device__ inline void stage_store(const StageRegs<StoreT>& r, const StripGeom& 
         const int e = t + it * kBlurThreads;
         if (e >= kVecPerBlock) continue;
         const int row = e / kVecPerRow, v = e - row * kVecPerRow;
-        const int gy = g.ty0 - kHalo + 16 * kb + row, gx = g.tx0 - kHalo + 4 * v;
-        const bool inside = gy >= 0 && gy < g.h && gx >= 0;
         float av[4], bv[4];
         unpack4(r.a[it], av);
         unpack4(r.b[it], bv);
         float va[4], vb[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const bool in = inside && gx + i < g.w;
-            const float A = fmaf(av[i], s, -cs);
-            const float B = (sizeof(StoreT) == 2) ? fmaf(bv[i], s, A) : fmaf(bv[i], s, -cs);
-            va[i] = in ? A : 0.0f;
-            vb[i] = in ? B : 0.0f;
+            va[i] = fmaf(av[i], s, -cs);
+            vb[i] = (sizeof(StoreT) == 2) ? fmaf(bv[i], s, va[i]) : fmaf(bv[i], s, -cs);
         }
         const int off = (kb % kSlots) * kSlotFloats + row * kRP + 4 * v;
         *reinterpret_cast<float4*>(sA + off) = make_float4(va[0], va[1], va[2], va[3]);
         *reinterpret_cast<float4*>(sB + off) = make_float4(vb[0], vb[1], vb[2], vb[3]);
     }
-}
-
-// Accumulates the three hi/lo products of one 16x16x16 step onto c.
-__device__ inline floatx4 mma16_acc(const half4& ah, const half4& al, const half4& bh, const half4& bl, floatx4 c)
-{
-    c = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh, c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh, c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl, c, 0, 0, 0);
-    return c;
 }
 
 // Two values as hi + lo FP16 (hi rounded toward zero, lo = fp16(x - hi)).
@@ -381,10 +400,9 @@ __device__ inline void split_pair(float x0, float x1, float one, _Float16& h0, _
 // chosen so that |A|, |B| < 2^6 (so |A B| < 2^12 and (A - B)^2 < 2^14 stay in
 // FP16's range and hi/lo keep 22 significant bits; the subnormal floor of lo
 // is 2^-24 / s^2 of the item's magnitude, far below anything the maps
-// resolve). With one scale the two vertical products of an output block
-// accumulate in one MFMA chain. The blurred quantities are A, B, A B and
-// (A - B)^2, and sigma1^2 + sigma2^2 = 2 sigma12 + (E[(A - B)^2] - (mu1 - mu2)^2):
-// for a == b the last term is 0 exactly, so the SSIM ratio is 1 bit for bit.
+// resolve). The blurred quantities are A, B, A B and (A - B)^2, and
+// sigma1^2 + sigma2^2 = 2 sigma12 + (E[(A - B)^2] - (mu1 - mu2)^2): for a == b
+// the last term is 0 exactly, so the SSIM ratio is 1 bit for bit.
 // The maps are evaluated on the scaled
 // quantities: the SSIM ratio's numerator and denominator both carry s^2
 // exactly (C2 becomes C2 s^2, (mu1 - mu2)^2 is unscaled by s^-2) and the edge
@@ -392,12 +410,14 @@ __device__ inline void split_pair(float x0, float x1, float one, _Float16& h0, _
 // so the maps are those of the unscaled values bit for bit. Outputs outside
 // the image have zero normalisers and zero samples, which makes both maps 0
 // exactly (the SSIM ratio is C2 s^2 / C2 s^2 = 1, the edge ratio s / s), so
-// no output is masked by a branch.
+// no output is masked by a branch. Both edge maps are always summed (the
+// scorers read only the maps a job's flags select); NQ = 4 jobs add SSIM.
 template <typename StoreT, int NQ>
 __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, const BlurTaps& taps,
                           const ToeplitzLds* TL, float* sA, float* sB, float* red, float* rowinv,
                           float* __restrict__ partial)
 {
+    typedef typename StageRegs<StoreT>::Vec Vec;
     const int t = threadIdx.x;
     const int wave = t >> 6, lane = t & 63;
     const int ln = lane & 15, lg = lane >> 4;
@@ -409,8 +429,17 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
     g.ty0 = tile_y * 16 * segb;
     const uint32_t c_at = (uint32_t)g.ty0 * (uint32_t)g.pitch + (uint32_t)g.tx0;
     float c;
-    if constexpr (sizeof(StoreT) == 4) c = global_ptr<float>(job.a)[c_at];
-    else c = __half2float(__ushort_as_half(global_ptr<unsigned short>(job.a)[c_at]));
+    Vec fa, fb;
+    if constexpr (sizeof(StoreT) == 4) {
+        c = global_ptr<float>(job.a)[c_at];
+        fa = Vec{c, c, c, c};
+        fb = fa;
+    } else {
+        const uint32_t cb = global_ptr<unsigned short>(job.a)[c_at];
+        c = __half2float(__ushort_as_half((unsigned short)cb));
+        fa = Vec{cb | (cb << 16), cb | (cb << 16)};
+        fb = Vec{0u, 0u};
+    }
     // output row blocks of this item that hold image rows; window blocks = that + 1
     const int nrb = min(segb, (g.h - g.ty0 + 15) / 16);
     const int nwb = nrb + 1;
@@ -418,18 +447,29 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
     // 1 / (in-bounds tap sum) of each output row of the item, 0 below the image
     if (t < 16 * segb) rowinv[t] = inv_norm(g.ty0 + t, g.h, taps);
 
-    // the item's scale: largest centred magnitude over the whole window (the
-    // blocks in reverse order, so block 0 is the one left in the registers)
+    // the item's scale from a bound over the whole window: two blocks per
+    // step so that their loads are in flight together, in reverse order so
+    // that block 0 is the one left in st
     StageRegs<StoreT> st;
-    uint32_t mbits = 0;
-    for (int kb = nwb - 1; kb >= 0; kb--) {
-        stage_load<StoreT>(job.a, job.b, g, kb, st);
-        mbits = max(mbits, stage_max_bits<StoreT>(st, g, kb, c));
+    {
+        WindowBound<StoreT> bnd(c);
+        StageRegs<StoreT> st2;
+        for (int kb = nwb - 1; kb > 0; kb -= 2) {
+            stage_load<StoreT>(job.a, job.b, g, kb, st2, fa, fb);
+            stage_load<StoreT>(job.a, job.b, g, kb - 1, st, fa, fb);
+            bnd.add(st2, c);
+            bnd.add(st, c);
+        }
+        if (nwb & 1) {
+            stage_load<StoreT>(job.a, job.b, g, 0, st, fa, fb);
+            bnd.add(st, c);
+        }
+        const uint32_t mbits = wave_max_bits(__float_as_uint(bnd.value()));
+        uint32_t* redu = reinterpret_cast<uint32_t*>(red);
+        if (lane == 0) redu[wave] = mbits;
     }
-    mbits = wave_max_bits(mbits);
-    uint32_t* redu = reinterpret_cast<uint32_t*>(red);
-    if (lane == 0) redu[wave] = mbits;
     __syncthreads();
+    const uint32_t* redu = reinterpret_cast<const uint32_t*>(red);
     const float vmax = __uint_as_float(max(max(redu[0], redu[1]), max(redu[2], redu[3])));
     int e1 = 6;
     if (vmax > 0.0f) {
@@ -443,22 +483,30 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
     const float inv_s2 = ldexpf(1.0f, -2 * e1);
     const float c2s2 = kC2 * ldexpf(1.0f, 2 * e1);
 
-    stage_store<StoreT>(st, g, 0, s, cs, sA, sB);
+    stage_store<StoreT>(st, 0, s, cs, sA, sB);
     __syncthreads();
 
     const int x = g.tx0 + 16 * wave + ln;
     const float inx = inv_norm(x, g.w, taps);  // 0 at or beyond w
     float one = 1.0f;
     asm volatile("" : "+s"(one));  // emits nothing: keeps fma(x, one, -hi) an FMA
-    const bool do_ssim = (job.flags & kMapSsim) != 0;
-    const bool do_art = (job.flags & kMapArtifact) != 0, do_det = (job.flags & kMapDetail) != 0;
-    const bool do_edge = do_art || do_det;
     float s1m[3] = {0.f, 0.f, 0.f}, s4m[3] = {0.f, 0.f, 0.f};
-    floatx4 acc[NQ];  // vertical accumulation of the output block in progress
+    // hi/lo horizontal results of two consecutive window blocks per quantity:
+    // block kb sits in elements 4 (kb & 1) .. +3, so the vertical B operand
+    // of output block kb - 1 is the register tuple itself, read in the K order
+    // of the table for kb's parity (the loop runs two blocks per turn so that
+    // the parity is a constant)
+    half8 xh[NQ], xl[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        xh[q] = half8{0, 0, 0, 0, 0, 0, 0, 0};
+        xl[q] = xh[q];
+    }
 
-    for (int kb = 0; kb < nwb; kb++) {
+    auto step = [&](auto parity, int kb) {
+        constexpr int P = decltype(parity)::value;
         const bool more = kb + 1 < nwb;
-        if (more) stage_load<StoreT>(job.a, job.b, g, kb + 1, st);
+        if (more) stage_load<StoreT>(job.a, job.b, g, kb + 1, st, fa, fb);
 
         // ---- horizontal operand: window row 16kb + ln, columns 16w + 8lg .. +7
         const float* ra = sA + (kb % kSlots) * kSlotFloats + ln * kRP + 16 * wave + 8 * lg;
@@ -473,7 +521,6 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
 
         // ---- per quantity (A, B, A B, (A - B)^2): hi/lo operand, horizontal
         // product, hi/lo of the results (the vertical B operand)
-        half4 bh[NQ], bl[NQ];
         const half8 g_hi = TL->g_hi[lane], g_lo = TL->g_lo[lane];
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
@@ -498,16 +545,25 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
             }
             const floatx4 hq = mma32(ah, al, g_hi, g_lo);
             const float hv[4] = {hq[0], hq[1], hq[2], hq[3]};
-            split4(hv, one, bh[q], bl[q]);
+            half4 bh, bl;
+            split4(hv, one, bh, bl);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                xh[q][4 * P + j] = bh[j];
+                xl[q][4 * P + j] = bl[j];
+            }
         }
 
-        // ---- output block kb - 1 is complete: T1 part, maps
+        // ---- output block kb - 1: vertical product over blocks kb - 1 and kb, maps
         if (kb >= 1) {
             const int rbk = kb - 1;
-            half4 t1_hi, t1_lo;
-            split_half8(TL->t1[lane], t1_hi, t1_lo);
+            floatx4 acc[NQ];
+            {
+                const half8 tv_hi = P ? TL->tv_hi[lane] : TL->tw_hi[lane];
+                const half8 tv_lo = P ? TL->tv_lo[lane] : TL->tw_lo[lane];
 #pragma unroll
-            for (int q = 0; q < NQ; q++) acc[q] = mma16_acc(t1_hi, t1_lo, bh[q], bl[q], acc[q]);
+                for (int q = 0; q < NQ; q++) acc[q] = mma32(tv_hi, tv_lo, xh[q], xl[q]);
+            }
             // centred scaled samples at the outputs: window rows 16rbk + 8 + 4lg + i
             const float* ca;
             const float* cb;
@@ -525,46 +581,37 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
                 const float nrm = inx * rnv[i];
                 const float m1 = acc[0][i] * nrm, m2 = acc[1][i] * nrm;
                 if constexpr (NQ == 4) {
-                    if (do_ssim) {
 #pragma clang fp contract(off)
-                        const float s12 = acc[2][i] * nrm;
-                        const float dsq = acc[3][i] * nrm;
-                        const float md = m1 - m2, md2 = md * md;
-                        const float num_m = 1.0f - md2 * inv_s2;
-                        const float num_s = 2.0f * (s12 - m1 * m2) + c2s2;
-                        // sigma1^2 + sigma2^2 = 2 sigma12 + var(A - B)
-                        const float den_s = num_s + (dsq - md2);
-                        const float val = fmaxf(1.0f - div_nr(num_m * num_s, den_s), 0.0f);
-                        const float v2 = val * val;
-                        s1m[0] += val;
-                        s4m[0] = fmaf(v2, v2, s4m[0]);
-                    }
+                    const float s12 = acc[2][i] * nrm;
+                    const float dsq = acc[3][i] * nrm;
+                    const float md = m1 - m2, md2 = md * md;
+                    const float num_m = 1.0f - md2 * inv_s2;
+                    const float num_s = 2.0f * (s12 - m1 * m2) + c2s2;
+                    // sigma1^2 + sigma2^2 = 2 sigma12 + var(A - B)
+                    const float den_s = num_s + (dsq - md2);
+                    const float val = fmaxf(1.0f - div_nr(num_m * num_s, den_s), 0.0f);
+                    const float v2 = val * val;
+                    s1m[0] += val;
+                    s4m[0] = fmaf(v2, v2, s4m[0]);
                 }
-                if (do_edge) {
+                {
 #pragma clang fp contract(off)
                     const float dd = div_nr(fabsf(cb[i * kRP] - m2) + s, fabsf(ca[i * kRP] - m1) + s) - 1.0f;
-                    if (do_art) {
-                        const float art = fmaxf(dd, 0.0f), a2 = art * art;
-                        s1m[1] += art;
-                        s4m[1] = fmaf(a2, a2, s4m[1]);
-                    }
-                    if (do_det) {
-                        const float det = fmaxf(-dd, 0.0f), d2 = det * det;
-                        s1m[2] += det;
-                        s4m[2] = fmaf(d2, d2, s4m[2]);
-                    }
+                    const float art = fmaxf(dd, 0.0f), a2 = art * art;
+                    const float det = fmaxf(-dd, 0.0f), d2 = det * det;
+                    s1m[1] += art;
+                    s4m[1] = fmaf(a2, a2, s4m[1]);
+                    s1m[2] += det;
+                    s4m[2] = fmaf(d2, d2, s4m[2]);
                 }
             }
         }
-        // ---- T0 part of output block kb (starts its accumulation)
-        if (kb < nrb) {
-            half4 t0_hi, t0_lo;
-            split_half8(TL->t01[lane], t0_hi, t0_lo);
-#pragma unroll
-            for (int q = 0; q < NQ; q++) acc[q] = mma16(t0_hi, t0_lo, bh[q], bl[q]);
-        }
-        if (more) stage_store<StoreT>(st, g, kb + 1, s, cs, sA, sB);
+        if (more) stage_store<StoreT>(st, kb + 1, s, cs, sA, sB);
         __syncthreads();
+    };
+    for (int kb = 0; kb < nwb; kb += 2) {
+        step(std::integral_constant<int, 0>{}, kb);
+        if (kb + 1 < nwb) step(std::integral_constant<int, 1>{}, kb + 1);
     }
 
     // ---- workgroup reduction of the 6 sums
@@ -586,7 +633,9 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
 }
 
 template <typename StoreT>
-// 4 waves per SIMD: 128 registers per lane, 36.4 KB of LDS per workgroup.
+// 4 waves per SIMD: 128 registers per lane, 38.6 KB of LDS per workgroup
+// (measured faster than 3 waves per SIMD without the spills 128 registers
+// cost).
 __global__ __launch_bounds__(kBlurThreads, 4)
 void blur_strip_kernel(const MapJob* __restrict__ jobs, const int* __restrict__ counts, BlurTaps taps, int segb,
                        float* __restrict__ partials)
