@@ -2,10 +2,11 @@
 reduced size: 6 sources at 1920x1080 (8-bit YUV 4:2:0) x 16 DCT-quantised
 candidates, target score 70. The surrogate-screened selection must equal the
 exact-only selection; every score it confirms must equal the exact batched
-score of the same candidate bit for bit (the confirmation completes scales
-3-4 on the planes the feature pass stored); and a fallback threshold forced to
-0 sends every candidate off the shortlist to the exact metric and still
-agrees."""
+score of the same candidate bit for bit (the confirmation completes the
+scales outside the network's inputs on the planes the feature pass stored);
+and a fallback threshold forced to 0 sends every candidate off the shortlist
+to the exact metric and still agrees. Both shipped networks: inputs from
+scales 0-2 and from scales 2-4."""
 import numpy as np
 import pytest
 
@@ -29,10 +30,13 @@ def screening():
     scr.close()
 
 
-def test_surrogate_screening_agrees_with_exact_only(screening):
+@pytest.mark.parametrize("params,scales", [(SG.PARAMS_FILE, SG.FEATURE_SCALES),
+                                           (SG.COARSE_PARAMS_FILE, SG.COARSE_SCALES)], ids=["scales012", "scales234"])
+def test_surrogate_screening_agrees_with_exact_only(screening, params, scales):
     ncand = screening.nsrc * screening.ncand
     with SG.SurrogateNet(max_batch=ncand) as net, S.SSIMU2Batch(ncand, S.STORAGE_FP16) as bt:
-        net.load(SG.PARAMS_FILE)
+        net.load(params)
+        assert net.feature_scales == scales
         exact_choice, exact_scores = screening.exact_only(bt, TARGET)
         choice, stats, mean, sigma = screening.with_surrogate(bt, net, TARGET)
         _say("exact", exact_choice, "surrogate", choice, "confirmed", stats["confirmed_pairs"],

@@ -37,13 +37,17 @@ struct MapSums {
     float s4[3];  // sum of their 4th powers
 };
 
-// SSIM map value, reference src/ssimu2/score.hpp:266-275, with
-// s11 + s22 = G(a^2 + b^2) blurred directly instead of G((a+b)^2) - 2 G(ab).
-// All inputs may be centred on one constant (m = mu - c, s12 = G((a-c)(b-c)),
-// sq = G((a-c)^2 + (b-c)^2)): the formula only uses mu1 - mu2 and the
-// (co)variances, which centring leaves unchanged.
-// For a == b: sq == 2 * s12 and m11 + m22 == 2 * m12 exactly, so num_s and
-// den_s are the same float and the map is exactly 0.
+// SSIM map value 1 - num_m num_s / den_s, reference src/ssimu2/score.hpp:266-275,
+// with s11 + s22 = G(a^2 + b^2) blurred directly instead of
+// G((a+b)^2) - 2 G(ab). All inputs may be centred on one constant
+// (m = mu - c, s12 = G((a-c)(b-c)), sq = G((a-c)^2 + (b-c)^2)): the formula
+// only uses mu1 - mu2 and the (co)variances, which centring leaves unchanged.
+// Evaluated as 1 - (1 - u)(1 - v) = u + v - u v with u = (mu1 - mu2)^2 and
+// v = var(a - b) / (s11 + s22 + C2), var(a - b) = s11 + s22 - 2 s12: the same
+// value, formed from the two small quantities instead of a difference from 1
+// that float resolves only to 2^-24 (near-identical images have maps of that
+// order). For a == b: sq == 2 * s12 and m11 + m22 == 2 * m12 exactly, so u and
+// var(a - b) are 0 and so is the map.
 __device__ inline float ssim_value(float m1, float m2, float s12, float sq)
 {
 #pragma clang fp contract(off)
@@ -51,10 +55,11 @@ __device__ inline float ssim_value(float m1, float m2, float s12, float sq)
     const float m22 = m2 * m2;
     const float m12 = m1 * m2;
     const float md = m1 - m2;
-    const float num_m = 1.0f - md * md;
-    const float num_s = 2.0f * (s12 - m12) + kC2;
-    const float den_s = (sq - (m11 + m22)) + kC2;
-    return fmaxf(1.0f - div_nr(num_m * num_s, den_s), 0.0f);
+    const float u = md * md;
+    const float s_sum = sq - (m11 + m22);                // s11 + s22
+    const float var = s_sum - 2.0f * (s12 - m12);        // s11 + s22 - 2 s12
+    const float v = div_nr(var, s_sum + kC2);
+    return fmaxf(fmaf(-u, v, u + v), 0.0f);
 }
 
 // Edge difference d = (|b - mu2| + 1) / (|a - mu1| + 1) - 1, artifact =

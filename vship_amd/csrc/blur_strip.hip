@@ -401,8 +401,8 @@ __device__ inline void split_pair(float x0, float x1, float one, _Float16& h0, _
 // FP16's range and hi/lo keep 22 significant bits; the subnormal floor of lo
 // is 2^-24 / s^2 of the item's magnitude, far below anything the maps
 // resolve). The blurred quantities are A, B, A B and (A - B)^2, and
-// sigma1^2 + sigma2^2 = 2 sigma12 + (E[(A - B)^2] - (mu1 - mu2)^2): for a == b
-// the last term is 0 exactly, so the SSIM ratio is 1 bit for bit.
+// sigma1^2 + sigma2^2 = 2 sigma12 + var(A - B), var(A - B) =
+// E[(A - B)^2] - (mu1 - mu2)^2: for a == b it is 0 exactly and so is the map.
 // The maps are evaluated on the scaled
 // quantities: the SSIM ratio's numerator and denominator both carry s^2
 // exactly (C2 becomes C2 s^2, (mu1 - mu2)^2 is unscaled by s^-2) and the edge
@@ -584,12 +584,16 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
 #pragma clang fp contract(off)
                     const float s12 = acc[2][i] * nrm;
                     const float dsq = acc[3][i] * nrm;
+                    // 1 - num_m num_s / den_s = u + v - u v with
+                    // u = (mu1 - mu2)^2, v = var(A - B) / den_s and
+                    // den_s = 2 sigma12 + C2 + var(A - B): formed from the
+                    // small quantities, not as a difference from 1
                     const float md = m1 - m2, md2 = md * md;
-                    const float num_m = 1.0f - md2 * inv_s2;
+                    const float u = md2 * inv_s2;
+                    const float var = dsq - md2;
                     const float num_s = 2.0f * (s12 - m1 * m2) + c2s2;
-                    // sigma1^2 + sigma2^2 = 2 sigma12 + var(A - B)
-                    const float den_s = num_s + (dsq - md2);
-                    const float val = fmaxf(1.0f - div_nr(num_m * num_s, den_s), 0.0f);
+                    const float v = div_nr(var, num_s + var);
+                    const float val = fmaxf(fmaf(-u, v, u + v), 0.0f);
                     const float v2 = val * val;
                     s1m[0] += val;
                     s4m[0] = fmaf(v2, v2, s4m[0]);

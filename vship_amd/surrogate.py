@@ -1,6 +1,10 @@
-"""Score surrogate: an MLP that predicts the SSIMULACRA2 score from the 54
-features of scales 0-2 plus log(pixels), with an uncertainty head, trained on
-the GPU by the library's own kernels (VshipAMD_MLP*, vship_amd/csrc/mlp.hip).
+"""Score surrogate: an MLP that predicts the SSIMULACRA2 score from 54
+features (those of three of the five scales) plus log(pixels), with an
+uncertainty head, trained on the GPU by the library's own kernels
+(VshipAMD_MLP*, vship_amd/csrc/mlp.hip). Two networks ship: one on the
+scale 0-2 features (data/surrogate.npz) and one on the scale 2-4 features
+(data/surrogate_coarse.npz), whose inputs cost a small part of the blur, so
+that screening with it is faster than exact-only screening.
 
 Pieces:
 * ``SurrogateNet`` - ctypes owner of one network (BF16 weights, FP32 math).
@@ -36,10 +40,29 @@ from scipy import ndimage
 from . import ssimu2 as S
 from . import synth
 
-PARAMS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate.npz")
+_DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+PARAMS_FILE = os.path.join(_DATA, "surrogate.npz")  # inputs: scale 0-2 features
+COARSE_PARAMS_FILE = os.path.join(_DATA, "surrogate_coarse.npz")  # inputs: scale 2-4 features
 N_IN, N_HID, N_OUT = 55, 128, 2
-# 108-feature index plane * 36 + scale * 6 + k of the 54 inputs (scales 0-2)
-FEATURE_INDEX = np.array([p * 36 + s * 6 + k for p in range(3) for s in range(3) for k in range(6)])
+# Scale masks of the networks' inputs. Scales 0-2 hold 98% of the pyramid's
+# pixels; scales 2-4 hold 6%, so a screening pass that computes them for
+# every candidate costs the front-end and a small part of the blur, and
+# confirming a candidate exactly then blurs its scales 0-1 from the stored
+# planes.
+FEATURE_SCALES = 0b00111
+COARSE_SCALES = 0b11100
+
+
+def feature_index(scales: int = FEATURE_SCALES) -> np.ndarray:
+    """Indices plane * 36 + scale * 6 + k into the 108 features of the 54
+    inputs of the scales in the mask (three of the five)."""
+    idx = [p * 36 + s * 6 + k for p in range(3) for s in range(5) if scales >> s & 1 for k in range(6)]
+    if len(idx) != 54:
+        raise ValueError("a network's inputs are the features of three scales")
+    return np.array(idx)
+
+
+FEATURE_INDEX = feature_index(FEATURE_SCALES)
 
 
 def _bind(lib):
@@ -69,11 +92,12 @@ def _bind(lib):
     return lib
 
 
-def inputs_from_features(features108, pixels) -> np.ndarray:
-    """(n, 55) float32 network inputs: scale 0-2 features and log(pixels)."""
+def inputs_from_features(features108, pixels, scales: int = FEATURE_SCALES) -> np.ndarray:
+    """(n, 55) float32 network inputs: the features of the scales in the mask
+    and log(pixels)."""
     f = np.asarray(features108, dtype=np.float64).reshape(-1, 108)
     x = np.empty((f.shape[0], N_IN), dtype=np.float32)
-    x[:, :54] = f[:, FEATURE_INDEX]
+    x[:, :54] = f[:, feature_index(scales)]
     x[:, 54] = np.log(np.asarray(pixels, dtype=np.float64))
     return x
 
@@ -122,6 +146,7 @@ class SurrogateNet:
         self.max_batch = max_batch
         self.param_count = self._lib.VshipAMD_MLPParamCount()
         self.norm = None
+        self.feature_scales = FEATURE_SCALES  # scale mask of the inputs (set by load)
 
     def set_params(self, params) -> None:
         p = np.ascontiguousarray(params, dtype=np.float32)
@@ -192,6 +217,7 @@ class SurrogateNet:
     def save(self, path: str, **meta) -> None:
         xm, xs, ym, ys = self.norm
         os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        meta = dict(meta, feature_scales=int(self.feature_scales))
         np.savez(path, params=self.params(), xmean=xm, xstd=xs, ymean=ym, ystd=ys,
                  meta=np.array(json.dumps(meta)))
 
@@ -199,7 +225,9 @@ class SurrogateNet:
         with np.load(path, allow_pickle=False) as z:
             self.set_params(z["params"])
             self.set_norm(z["xmean"], z["xstd"], float(z["ymean"]), float(z["ystd"]))
-            return json.loads(str(z["meta"]))
+            meta = json.loads(str(z["meta"]))
+        self.feature_scales = int(meta.get("feature_scales", FEATURE_SCALES))
+        return meta
 
     def close(self) -> None:
         if getattr(self, "_open", False):
@@ -426,12 +454,6 @@ def screening_set(sources: int = 64, candidates: int = 16, width: int = 1920, he
     return refs, dists, qs
 
 
-# scale masks of the two screening passes: the network's inputs are the
-# scale 0-2 features; confirming a candidate exactly adds scales 3-4
-FEATURE_SCALES = 0b00111
-REST_SCALES = S.ALL_SCALES & ~FEATURE_SCALES
-
-
 def _select(scores, qs, target):
     """Highest-q candidate scoring at least `target`; the best-scoring one
     if none does."""
@@ -500,22 +522,24 @@ class Screening:
     def with_surrogate(self, batch, net, target, top=4, fallback_sigma=1.5):
         """Surrogate-screened selection. Returns (choices, stats, mean, sigma).
 
-        One batched call blurs scales 0-2 of every candidate (the network's
-        inputs; the planes of scales 3-4 are stored on the way). The network
+        One batched call blurs the scales of the network's inputs for every
+        candidate (the planes of the other scales are stored on the way). The network
         ranks every candidate; per source the `top` highest-q candidates
         predicted to reach the target (margin: two predicted standard
         deviations) plus every candidate whose predicted standard deviation
         exceeds fallback_sigma are confirmed by the exact metric, which only
-        has scales 3-4 left to blur for them (their exact scores equal a full
+        has the other scales left to blur for them (their exact scores equal a full
         evaluation's bit for bit); a source whose shortlist has no candidate
         reaching the target is completed for all its candidates and selected
         exactly. stats["confirmed"] maps (source, candidate) to the exact
         scores used."""
         items = [(i, k) for i in range(self.nsrc) for k in range(self.ncand)]
+        scales = net.feature_scales
+        rest_scales = S.ALL_SCALES & ~scales
         t0 = time.perf_counter()
-        _, feats = self._run(batch, items, FEATURE_SCALES)
+        _, feats = self._run(batch, items, scales)
         t1 = time.perf_counter()
-        X = inputs_from_features(feats, np.full(len(items), float(self.pixels)))
+        X = inputs_from_features(feats, np.full(len(items), float(self.pixels)), scales)
         mean, sigma = net.predict(X)
         t2 = time.perf_counter()
         mean = mean.reshape(self.nsrc, self.ncand)
@@ -531,7 +555,7 @@ class Screening:
             fallback_items += len(unsure)
             short.append(sorted(set(likely) | set(unsure)))
         confirm = [(i, k) for i in range(self.nsrc) for k in short[i]]
-        cscores = batch.complete_scales([i * self.ncand + k for i, k in confirm], REST_SCALES)
+        cscores = batch.complete_scales([i * self.ncand + k for i, k in confirm], rest_scales)
         exact = dict(zip(confirm, cscores))
         choices, full_fallback = [], []
         for i in range(self.nsrc):
@@ -544,7 +568,7 @@ class Screening:
         if full_fallback:
             rest = [(i, k) for i in full_fallback for k in range(self.ncand) if (i, k) not in exact]
             if rest:
-                rs = batch.complete_scales([i * self.ncand + k for i, k in rest], REST_SCALES)
+                rs = batch.complete_scales([i * self.ncand + k for i, k in rest], rest_scales)
                 exact.update(zip(rest, rs))
             for i in full_fallback:
                 choices[i] = _select([exact[(i, k)] for k in range(self.ncand)], self.qs, target)
@@ -598,17 +622,23 @@ def _train_main(argv):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--fallback-sigma", type=float, default=1.5)
-    ap.add_argument("--out", default=PARAMS_FILE)
+    ap.add_argument("--feature-scales", type=lambda v: int(v, 0), default=FEATURE_SCALES,
+                    help="scale mask of the inputs: 0b00111 (scales 0-2) or 0b11100 (scales 2-4)")
+    ap.add_argument("--out", default=None, help="default: data/surrogate.npz or data/surrogate_coarse.npz")
     ap.add_argument("--report", default=None)
     a = ap.parse_args(argv)
+    feature_index(a.feature_scales)  # three scales
+    if a.out is None:
+        a.out = PARAMS_FILE if a.feature_scales == FEATURE_SCALES else COARSE_PARAMS_FILE
     log = lambda m: print(m, flush=True)
     t0 = time.perf_counter()
     feats, scores, pixels, dparams = generate(a.pairs, a.seed, workers=a.workers, log=log, processes=True)
     t_data = time.perf_counter() - t0
-    X = inputs_from_features(feats, pixels)
+    X = inputs_from_features(feats, pixels, a.feature_scales)
     y = scores.astype(np.float32)
     ntr = a.pairs - a.holdout
     with SurrogateNet(max_batch=a.batch) as net:
+        net.feature_scales = a.feature_scales
         net.fit_norm(X[:ntr], y[:ntr])
         net.set_params(init_params(a.seed))
         t1 = time.perf_counter()
@@ -619,7 +649,7 @@ def _train_main(argv):
         train = evaluate(net, X[:ntr][:20000], y[:ntr][:20000], a.fallback_sigma)
         report = {
             "pairs": a.pairs, "train_pairs": ntr, "holdout_pairs": a.holdout, "epochs": a.epochs,
-            "batch": a.batch, "lr": a.lr, "seed": a.seed,
+            "batch": a.batch, "lr": a.lr, "seed": a.seed, "feature_scales": a.feature_scales,
             "data_seconds": round(t_data, 2), "train_seconds": round(t_train, 3),
             "loss_per_epoch": [round(float(v), 5) for v in losses],
             "holdout": test, "train_subset": train,
@@ -639,7 +669,8 @@ def _train_main(argv):
 def main(argv=None):
     argv = sys.argv[1:] if argv is None else argv
     if not argv or argv[0] != "train":
-        print("usage: python -m vship_amd.surrogate train [--pairs N] [--epochs E] [--out FILE] [--report FILE]")
+        print("usage: python -m vship_amd.surrogate train [--pairs N] [--epochs E] [--feature-scales MASK] "
+              "[--out FILE] [--report FILE]")
         return 2
     return _train_main(argv[1:])
 
