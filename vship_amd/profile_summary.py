@@ -11,6 +11,13 @@ python -m vship_amd.profile_summary valu KERNEL_DB SIMDS XCDS
     (a wave64 VALU instruction holds a 16-lane SIMD for 4 cycles) summed over
     the chip, GRBM_GUI_ACTIVE counts the dispatch's cycles once per XCD, so
     busy = 4 * SQ_ACTIVE_INST_VALU / (SIMDS * GRBM_GUI_ACTIVE / XCDS)
+python -m vship_amd.profile_summary counters OUT_JSON WORKLOAD PAIRS_PER_LAUNCH DB [DB...]
+    every SQ_/GRBM_ counter of the runs DB... (one counter group per run,
+    same workload) per kernel, mean per dispatch, with the derived issue
+    fractions (SIMDS = 1024, XCDS = 8): VALU busy as above, MFMA busy =
+    SQ_VALU_MFMA_BUSY_CYCLES / (SIMDS * GRBM_GUI_ACTIVE / XCDS), and VALU
+    lane-operations per pair = SQ_INSTS_VALU * 64 / PAIRS_PER_LAUNCH; merged
+    into OUT_JSON under WORKLOAD, and printed as a Markdown table
 python -m vship_amd.profile_summary busy RESULTS_DB WINDOW [NAME...]
     GPU occupancy of a kernel trace, per window of WINDOW consecutive
     dispatches of the kernels whose names contain one of NAME (all kernels
@@ -42,6 +49,42 @@ def pmc(counter: str, db: str) -> dict[str, float]:
     for name, _d, v in c.execute(q, (counter,)):
         acc.setdefault(_short(name), []).append(v)
     return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+SIMDS, XCDS = 1024, 8
+
+
+def counters(dbs: list[str]) -> dict[str, dict[str, float]]:
+    """{kernel: {counter: mean per dispatch}} over the runs (one group each)."""
+    out: dict[str, dict[str, float]] = {}
+    for db in dbs:
+        c = sqlite3.connect(db)
+        q = ("select kernel_name, counter_name, dispatch_id, sum(value) from counters_collection "
+             "group by dispatch_id, counter_name")
+        acc: dict[tuple[str, str], list[float]] = {}
+        for name, counter, _d, v in c.execute(q):
+            acc.setdefault((_short(name), counter), []).append(v)
+        for (k, counter), v in acc.items():
+            out.setdefault(k, {})[counter] = sum(v) / len(v)
+    return out
+
+
+def derived(cnt: dict[str, float], pairs_per_launch: float) -> dict[str, float]:
+    d = dict(cnt)
+    grbm = cnt.get("GRBM_GUI_ACTIVE")
+    if grbm:
+        cyc = grbm / XCDS
+        d["cycles"] = cyc
+        if "SQ_ACTIVE_INST_VALU" in cnt:
+            d["valu_busy"] = 4.0 * cnt["SQ_ACTIVE_INST_VALU"] / (SIMDS * cyc)
+        if "SQ_VALU_MFMA_BUSY_CYCLES" in cnt:
+            d["mfma_busy"] = cnt["SQ_VALU_MFMA_BUSY_CYCLES"] / (SIMDS * cyc)
+    if "SQ_INSTS_VALU" in cnt:
+        d["valu_insts_per_pair"] = cnt["SQ_INSTS_VALU"] / pairs_per_launch
+        d["valu_lane_ops_per_pair"] = cnt["SQ_INSTS_VALU"] * 64.0 / pairs_per_launch
+    if "SQ_INSTS_MFMA" in cnt:
+        d["mfma_insts_per_pair"] = cnt["SQ_INSTS_MFMA"] / pairs_per_launch
+    return d
 
 
 def busy(db: str, window: int, names: list[str]) -> list[tuple[int, float, float, float]]:
@@ -92,6 +135,22 @@ def main(argv=None) -> int:
                 continue
             cyc = grbm[k] / xcds
             print(f"| `{k}` | {ins[k]:.0f} | {act[k]:.0f} | {cyc:.0f} | {4.0 * act[k] / (simds * cyc):.3f} |")
+        return 0
+    if a and a[0] == "counters":
+        import json
+        import os
+        out_json, workload, ppl, dbs = a[1], a[2], float(a[3]), a[4:]
+        table = {k: derived(v, ppl) for k, v in sorted(counters(dbs).items()) if "copyBuffer" not in k}
+        doc = json.load(open(out_json)) if os.path.exists(out_json) else {}
+        doc[workload] = {"pairs_per_launch": ppl, "kernels": table}
+        with open(out_json, "w") as f:
+            json.dump(doc, f, indent=1, sort_keys=True)
+        cols = ["SQ_INSTS_VALU", "SQ_INSTS_MFMA", "valu_lane_ops_per_pair", "valu_busy", "mfma_busy", "cycles"]
+        print("| kernel | " + " | ".join(cols) + " |")
+        print("|---|" + "---|" * len(cols))
+        for k, d in table.items():
+            cells = [(f"{d[c]:.3f}" if c.endswith("busy") else f"{d[c]:,.0f}") if c in d else "" for c in cols]
+            print(f"| `{k}` | " + " | ".join(cells) + " |")
         return 0
     if a and a[0] == "busy":
         print("| first dispatch | span ms | GPU busy | kernels in flight (mean) |")

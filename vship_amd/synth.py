@@ -167,24 +167,53 @@ def mixed_counts(total: int):
     return counts
 
 
-def mixed_batch(counts, seed: int = 0, workers: int = 8):
+_SEQS = {}  # sequences of the jobs of frames_in_processes (inherited by forked workers)
+
+
+def _seq_frame(job):
+    key, i = job
+    seq = _SEQS[key]
+    ref, dist, q = seq.frame(i)
+    return seq.w, seq.h, seq.bits, ref, dist, q
+
+
+def frames_in_processes(seqs: dict, jobs, workers: int = 16):
+    """[(w, h, bits, ref, dist, q)] of seqs[key].frame(i) for (key, i) jobs,
+    in forked worker processes (numpy holds the GIL for much of a frame).
+    Call it before the process uses the GPU: the workers are forks."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+
+    _SEQS.clear()
+    _SEQS.update(seqs)
+    try:
+        with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("fork")) as ex:
+            return list(ex.map(_seq_frame, jobs, chunksize=max(1, len(jobs) // (4 * workers))))
+    finally:
+        _SEQS.clear()
+
+
+def mixed_batch(counts, seed: int = 0, workers: int = 8, processes: bool = False):
     """Seeded, shuffled list of (width, height, bits, ref planes, dist planes, q)
     with counts[k] pairs of MIXED_SIZES[k]; the content and distortion are those
-    of YuvSequence."""
+    of YuvSequence. processes: generate in forked worker processes (see
+    frames_in_processes), otherwise in threads."""
     import concurrent.futures as cf
 
-    jobs = []
+    seqs, jobs = {}, []
     for k, ((w, h, bits), n) in enumerate(zip(MIXED_SIZES, counts)):
         if n:
-            seq = YuvSequence(w, h, n, bits=bits, seed=seed + 101 * k)
-            jobs += [(seq, i) for i in range(n)]
+            seqs[k] = YuvSequence(w, h, n, bits=bits, seed=seed + 101 * k)
+            jobs += [(k, i) for i in range(n)]
+    if processes:
+        out = frames_in_processes(seqs, jobs, workers)
+    else:
+        def one(job):
+            k, i = job
+            ref, dist, q = seqs[k].frame(i)
+            return (seqs[k].w, seqs[k].h, seqs[k].bits, ref, dist, q)
 
-    def one(job):
-        seq, i = job
-        ref, dist, q = seq.frame(i)
-        return (seq.w, seq.h, seq.bits, ref, dist, q)
-
-    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
-        out = list(ex.map(one, jobs))
+        with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+            out = list(ex.map(one, jobs))
     order = np.random.default_rng(seed).permutation(len(out))
     return [out[i] for i in order]
