@@ -154,6 +154,56 @@ def test_device_entry_matches_host_entry():
     assert host == dev
 
 
+def test_pinned_host_planes_equal_device_planes():
+    """Vship_ComputeSSIMU2 on planes in pinned host memory (read by the
+    library's gather kernel), on pinned planes 4 bytes off a 16-byte boundary
+    and on pageable arrays (both copied by the copy engines) scores exactly as
+    the device-plane entry, over several calls of one handler."""
+    import ctypes
+    lib = S.load_library()
+    cs = S.colorspace(320, 180)
+    ref, dist, strides = cases.perturbed_pair(cs, seed=11, pad=0)
+    pinned = []
+
+    def pinned_planes(img, offset):
+        out = []
+        for p in img:
+            ptr = ctypes.c_void_p()
+            S.check(lib.Vship_PinnedMalloc(ctypes.byref(ptr), p.nbytes + 64))
+            pinned.append(ptr)
+            ctypes.memmove(ptr.value + offset, p.ctypes.data, p.nbytes)
+            out.append(ptr.value + offset)
+        return out
+
+    bufs, dptr = [], [[], []]
+    try:
+        for k, img in enumerate((ref, dist)):
+            for p in img:
+                b = S.DeviceBuffer(p.nbytes)
+                b.upload(p)
+                bufs.append(b)
+                dptr[k].append(b.address)
+        with S.SSIMU2Handler(cs, cs) as hd:
+            want = hd.compute_device(dptr[0], dptr[1], strides, strides)
+            got = {}
+            for offset in (0, 4):
+                p1, l1 = S._plane_args(pinned_planes(ref, offset), strides)
+                p2, l2 = S._plane_args(pinned_planes(dist, offset), strides)
+                for _ in range(3):
+                    sc = ctypes.c_double()
+                    S.check(lib.Vship_ComputeSSIMU2(hd.handle, ctypes.byref(sc), p1, p2, l1, l2))
+                    got.setdefault(offset, []).append(sc.value)
+            pageable = hd.compute(ref, dist, strides, strides)
+    finally:
+        for b in bufs:
+            b.free()
+        for p in pinned:
+            lib.Vship_PinnedFree(p)
+    _say("pinned host planes", want, got, pageable)
+    assert all(v == want for vals in got.values() for v in vals)
+    assert pageable == want
+
+
 def test_line_sizes_beyond_32bit_sample_offsets():
     """Line sizes of 2^24 bytes leave the 32-bit sample offsets of the
     layout-specific front-end; such calls run the generic front-end, and the

@@ -618,6 +618,33 @@ void Ssimu2Handler::run(const uint8_t* const p1[3], const uint8_t* const p2[3], 
     have_run_ = true;
 }
 
+namespace {
+
+// Device address of host bytes [p, p + bytes) when they lie in pinned host
+// memory the device can read (both ends registered, one contiguous mapping,
+// 16-byte aligned); null otherwise (pageable memory: the copy engines take
+// it). A failed query leaves no error for the next HIP call.
+const uint8_t* pinned_device_view(const uint8_t* p, size_t bytes)
+{
+    if (bytes == 0) return nullptr;
+    hipPointerAttribute_t a, b;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    if (a.type != hipMemoryTypeHost || a.devicePointer == nullptr) return nullptr;
+    if (hipPointerGetAttributes(&b, p + bytes - 1) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    const uint8_t* d = static_cast<const uint8_t*>(a.devicePointer);
+    if (b.type != hipMemoryTypeHost || static_cast<const uint8_t*>(b.devicePointer) != d + bytes - 1) return nullptr;
+    if (reinterpret_cast<uintptr_t>(d) % 16 != 0) return nullptr;
+    return d;
+}
+
+}  // namespace
+
 double Ssimu2Handler::compute_host(const uint8_t* const src1[3], const uint8_t* const src2[3], const int64_t ls1[3],
                                    const int64_t ls2[3])
 {
@@ -625,16 +652,38 @@ double Ssimu2Handler::compute_host(const uint8_t* const src1[3], const uint8_t* 
     const uint8_t* const* srcs[2] = {src1, src2};
     const int64_t* lss[2] = {ls1, ls2};
     const uint8_t* dev[2][3];
+    size_t bytes[2][3];
+    // planes in pinned host memory: one gather kernel reads all six over
+    // PCIe; otherwise one stream-ordered copy per plane
+    GatherPlanes g;
+    std::memset(&g, 0, sizeof(g));
+    bool direct = true;
     for (int i = 0; i < 2; i++) {
         ensure_staging(i, lss[i]);
         const ImageSpec& s = spec_[i];
         for (int p = 0; p < 3; p++) {
             if (srcs[i][p] == nullptr) VSHIP_THROW(Vship_BadPointer, "null plane pointer");
             // same extent as the reference's upload (vshipColor.hpp:96)
-            const size_t bytes = (size_t)(lss[i][p] * s.plane_h[p] - (lss[i][p] - s.bytes * s.plane_w[p]));
-            VSHIP_HIP_CHECK(hipMemcpyAsync(staging_[i][p], srcs[i][p], bytes, hipMemcpyHostToDevice, stream_));
+            bytes[i][p] = (size_t)(lss[i][p] * s.plane_h[p] - (lss[i][p] - s.bytes * s.plane_w[p]));
             dev[i][p] = staging_[i][p];
+            const uint8_t* view = direct ? pinned_device_view(srcs[i][p], bytes[i][p]) : nullptr;
+            if (view == nullptr) {
+                direct = false;
+            } else {
+                g.src[g.n] = view;
+                g.dst[g.n] = staging_[i][p];
+                g.bytes[g.n] = (int64_t)bytes[i][p];
+                g.n++;
+            }
         }
+    }
+    if (direct) {
+        VSHIP_HIP_CHECK(launch_gather(g, stream_));
+    } else {
+        for (int i = 0; i < 2; i++)
+            for (int p = 0; p < 3; p++)
+                VSHIP_HIP_CHECK(
+                    hipMemcpyAsync(staging_[i][p], srcs[i][p], bytes[i][p], hipMemcpyHostToDevice, stream_));
     }
     run(dev[0], dev[1], ls1, ls2);
     return *h_score_;

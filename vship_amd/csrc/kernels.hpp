@@ -88,4 +88,15 @@ hipError_t launch_score(const ScoreJob* pairs, const int* counts, int grid, cons
 // Plain 16-byte-vector copy (bandwidth probe and counter calibration).
 hipError_t launch_copy(void* dst, const void* src, int64_t bytes, hipStream_t stream);
 
+// Planes in pinned host memory (their device addresses) copied into device
+// buffers by one kernel; every address 16-byte aligned.
+constexpr int kGatherPlanes = 6;
+struct GatherPlanes {
+    const uint8_t* src[kGatherPlanes];
+    uint8_t* dst[kGatherPlanes];
+    int64_t bytes[kGatherPlanes];
+    int n;
+};
+hipError_t launch_gather(const GatherPlanes& g, hipStream_t stream);
+
 }  // namespace vship

@@ -5,6 +5,8 @@ python -m vship_amd.profile_workload batch --pairs 240   # batched FP16 path, 10
 python -m vship_amd.profile_workload batch --blur-config 5  # same, matrix-core blur
 python -m vship_amd.profile_workload batch --sources 16  # 240 pairs over 16 shared reference images
 python -m vship_amd.profile_workload copy --mib 256      # plain copy kernel (counter calibration)
+python -m vship_amd.profile_workload pcie --handlers 8 --pairs 16 --reps 15
+    # Vship_ComputeSSIMU2 on pinned host planes, one host thread per handler
 
 Each mode prints what it ran and the HIP-event time per kernel.
 """
@@ -13,6 +15,8 @@ from __future__ import annotations
 import argparse
 import ctypes
 import sys
+import threading
+import time
 
 import numpy as np
 
@@ -40,9 +44,65 @@ def _upload_pairs(n: int, seed: int = 0):
     return bufs, ptrs, strides
 
 
+def _pcie(a) -> int:
+    """a.pairs frame pairs in pinned host memory, a.reps passes over them by
+    a.handlers handlers (pair i on handler i mod n), one host thread each."""
+    lib = S.load_library()
+    seq = synth.YuvSequence(1920, 1080, a.pairs, seed=0)
+    pinned, pairs, strides = [], [], None
+    nbytes = 0
+    for i in range(a.pairs):
+        ref, dist, _ = seq.frame(i)
+        strides = S.host_strides(ref)
+        pair = []
+        for img in (ref, dist):
+            trip = []
+            for p in img:
+                ptr = ctypes.c_void_p()
+                S.check(lib.Vship_PinnedMalloc(ctypes.byref(ptr), p.nbytes))
+                ctypes.memmove(ptr, p.ctypes.data, p.nbytes)
+                pinned.append(ptr)
+                trip.append(ptr.value)
+                nbytes += p.nbytes if i == 0 else 0
+            pair.append(trip)
+        pairs.append((S._plane_args(pair[0], strides), S._plane_args(pair[1], strides)))
+    cs = S.colorspace(1920, 1080)
+    n = a.handlers
+    handlers = [S.SSIMU2Handler(cs, cs) for _ in range(n)]
+    total = a.pairs * a.reps
+
+    def worker(k, count):
+        sc = ctypes.c_double()
+        for i in range(k, count, n):
+            (p1, l1), (p2, l2) = pairs[i % len(pairs)]
+            S.check(lib.Vship_ComputeSSIMU2(handlers[k].handle, ctypes.byref(sc), p1, p2, l1, l2))
+
+    def run(count):
+        ths = [threading.Thread(target=worker, args=(k, count)) for k in range(n)]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+
+    try:
+        run(2 * n)  # each handler's first host-plane call allocates its staging buffers
+        t0 = time.perf_counter()
+        run(total)
+        dt = time.perf_counter() - t0
+    finally:
+        for h in handlers:
+            h.close()
+        for p in pinned:
+            lib.Vship_PinnedFree(p)
+    print(f"pcie: {n} handlers, {total} pairs of pinned host planes: {total / dt:.1f} pairs/s, "
+          f"{total * nbytes / dt / 1e9:.2f} GB/s host to device", flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["plain", "batch", "copy"])
+    ap.add_argument("mode", choices=["plain", "batch", "copy", "pcie"])
+    ap.add_argument("--handlers", type=int, default=8, help="pcie: handlers (one host thread each)")
     ap.add_argument("--pairs", type=int, default=8)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--mib", type=int, default=256)
@@ -62,6 +122,8 @@ def main(argv=None):
                                                        nbytes, a.reps, ctypes.byref(ms)))
         print(f"copy {nbytes} bytes: {ms.value:.4f} ms, {2 * nbytes / ms.value / 1e6:.1f} GB/s read+write", flush=True)
         return 0
+    if a.mode == "pcie":
+        return _pcie(a)
     bufs, ptrs, st = _upload_pairs(a.pairs)
     cs = S.colorspace(1920, 1080)
     print(f"{a.mode}: {a.pairs} pairs 1920x1080 yuv420p8, {a.reps} reps", flush=True)
