@@ -361,6 +361,39 @@ def run_plain_timed(handlers, call_args, scores, steps: int, warmup: int):
     return time.perf_counter() - t0
 
 
+DRIVER = os.path.join(ROOT, "vship_amd", "ffvship_amd")
+
+
+def driver_leg(frames, gpu_threads=(4, 8), reader_threads: int = 4):
+    """The throughput driver (FFVship's pipeline over raw YUV files: reader
+    threads fill pinned buffers, -g handler threads call Vship_ComputeSSIMU2)
+    on the 240 pairs written as two page-cached raw 1080p files. Runs before
+    this process initialises HIP; the driver is a child process."""
+    if not os.path.exists(DRIVER):
+        return {"skipped": f"{os.path.relpath(DRIVER, ROOT)} is not built"}
+    import tempfile
+    out = {"workload": f"{len(frames)} frame pairs 1920x1080 yuv420p (raw files, page cache), -t {reader_threads}",
+           "reference_context": "FFVship, README.md:204: 1339 frames in 9 s (148.8 fps) including video decode"}
+    with tempfile.TemporaryDirectory(prefix="vship_bench_") as d:
+        src, enc = os.path.join(d, "s.yuv"), os.path.join(d, "e.yuv")
+        with open(src, "wb") as fs, open(enc, "wb") as fe:
+            for ref, dist, _ in frames:
+                for p in ref:
+                    fs.write(np.ascontiguousarray(p).tobytes())
+                for p in dist:
+                    fe.write(np.ascontiguousarray(p).tobytes())
+        for g in gpu_threads:
+            cmd = [DRIVER, "--source", src, "--encoded", enc, "--width", str(W), "--height", str(H),
+                   "-g", str(g), "-t", str(reader_threads)]
+            r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=300)
+            if r.returncode != 0:
+                raise RuntimeError(f"driver run failed with {r.returncode}")
+            fps = [float(l.split(" at ")[1].split()[0]) for l in r.stdout.splitlines() if l.startswith("Computed")]
+            out[f"fps_g{g}"] = fps[0] if fps else None
+            log(f"driver -g {g} -t {reader_threads}: {out[f'fps_g{g}']} fps")
+    return out
+
+
 def pcie_leg(frames, strides, cs, handler_counts=(4, 8), n_host: int = 240):
     """Vship_ComputeSSIMU2 (the reference's call: planes in pinned host memory,
     copied to the device inside the call) with 4 and 8 handlers, one host
@@ -439,6 +472,7 @@ def main():
     ap.add_argument("--mixed-pairs", type=int, default=512)
     ap.add_argument("--no-screening", action="store_true")
     ap.add_argument("--no-pcie", action="store_true")
+    ap.add_argument("--no-driver", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "bench.py measures one GPU; frame pairs shard across GPUs with one "
@@ -463,6 +497,9 @@ def main():
     t0 = time.perf_counter()
     frames = make_frames(PAIRS)
     log(f"generated {PAIRS} pairs in {time.perf_counter() - t0:.1f}s ({WORKERS} worker processes)")
+    driver = None
+    if not args.plain_only and not args.no_driver:
+        driver = driver_leg(frames)
     mixed_data = None
     if not args.plain_only and not args.no_mixed:
         mixed_data = make_mixed(args.mixed_pairs)
@@ -610,6 +647,7 @@ def main():
         "batched_mixed_fp16": mixed,
         "surrogate_screening": screening,
         "pcie_inclusive": pcie,
+        "ffvship_driver": driver,
         "pcie_inclusive_pairs_per_s": pcie[str(args.handlers)]["pairs_per_s"] if pcie and str(args.handlers) in pcie
         else None,
         "mean_score": round(float(np.mean(scores)), 5),

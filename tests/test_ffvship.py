@@ -2,7 +2,8 @@
 files). CPU: frame selection against the examples of the reference's
 doc/FFVship.md ("encoded-offset", "source-indices"), argument errors. GPU:
 per-frame scores in --json and --live-score-output equal the C API's on the
-same frames (8-bit, 10-bit, cropped)."""
+same frames and are within the plain path's 0.005 of the FP64 oracle
+(8-bit, 10-bit, cropped)."""
 import json
 import os
 import subprocess
@@ -10,6 +11,7 @@ import subprocess
 import numpy as np
 import pytest
 
+import oracle
 from vship_amd import ssimu2 as S
 from vship_amd import synth
 
@@ -104,6 +106,9 @@ def test_driver_scores_equal_the_c_api(tmp_path, bits, crop):
     with S.SSIMU2Handler(cs, cs) as hd:
         want = [hd.compute(f[0], f[1], S.host_strides(f[0]), S.host_strides(f[1])) for f in frames]
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
+    exact = [oracle.ssimu2(cs, f[0], S.host_strides(f[0]), cs, f[1], S.host_strides(f[1]))[0] for f in frames]
+    print("driver vs oracle", max(abs(a - b) for a, b in zip(got, exact)), flush=True)
+    np.testing.assert_allclose(got, exact, rtol=0, atol=0.005)
     assert "Computed 6 frames" in p.stdout and "Average" in p.stdout
     live = run("--source", src, "--encoded", enc, "--width", w, "--height", h, "--pix-fmt", fmt, "-g", 3,
                "--live-score-output", *extra).stdout.strip().splitlines()

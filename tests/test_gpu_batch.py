@@ -6,6 +6,7 @@ import pytest
 
 import oracle
 from vship_amd import ssimu2 as S
+from vship_amd import synth
 from tests import cases
 
 pytestmark = pytest.mark.gpu
@@ -430,3 +431,80 @@ def test_strip_blur_keeps_22_bit_operands(pairs, config):
     rel = np.abs(g - w) / np.maximum(np.abs(w), 1e-3)
     _say(f"strip config {config}: max relative feature difference {rel.max():.2e}")
     np.testing.assert_allclose(g, w, rtol=1e-4, atol=1e-7)
+
+
+def test_batch_with_crop_and_resize_vs_oracle():
+    """Resized and cropped pairs beside plain ones in one batch (reference
+    vshipColor.hpp:87-103: linear RGB, cubic-Hermite resize, crop), including a
+    resampled reference shared by two pairs: FP32 storage with the handler's
+    blur items equals the single-pair handler's features bit for bit, and
+    both storages are within their bars of the FP64 oracle."""
+    def image(w, h, target, crop, seed, dist_q=None):
+        cs = S.colorspace(w, h)
+        cs.target_width, cs.target_height = target
+        cs.crop = S.CropRectangle(*crop)
+        seq = synth.YuvSequence(w, h, 2, seed=seed)
+        ref, dist, _ = seq.frame(0, q=dist_q if dist_q else 9.0)
+        return cs, ref, dist
+
+    cases_ = [  # (ref spec, dist spec): (w, h, (target_w, target_h), (top, bottom, left, right))
+        ((320, 180, (640, 360), (0, 0, 0, 0)), None),
+        ((640, 360, (256, 150), (4, 2, 6, 3)), None),
+        ((200, 120, (-1, -1), (3, 5, 7, 2)), None),
+        ((131, 75, (-1, -1), (0, 0, 0, 0)), None),        # plain
+        ((320, 180, (480, 270), (0, 0, 0, 0)), (480, 270, (-1, -1), (0, 0, 0, 0))),
+        ((640, 360, (-1, -1), (0, 0, 0, 0)), None),       # plain
+    ]
+    pairs_, bufs = [], []
+
+    def upload(planes):
+        out = []
+        for p in planes:
+            b = S.DeviceBuffer(p.nbytes)
+            b.upload(p)
+            bufs.append(b)
+            out.append(b.address)
+        return out
+
+    try:
+        for i, (rs, ds) in enumerate(cases_):
+            cs1, ref, dist = image(*rs, seed=30 + i)
+            if ds is None:
+                cs2 = cs1
+            else:
+                cs2, _, _ = image(*ds, seed=30 + i)
+                dist = synth.YuvSequence(ds[0], ds[1], 2, seed=30 + i).frame(0, q=9.0)[1]
+            pairs_.append((cs1, ref, upload(ref), cs2, dist, upload(dist)))
+        # a second pair on the first pair's resampled reference (same device planes)
+        cs1, ref, rptr, _, _, _ = pairs_[0]
+        d2 = synth.YuvSequence(320, 180, 2, seed=99).frame(0, q=20.0)[1]
+        pairs_.append((cs1, ref, rptr, cs1, d2, upload(d2)))
+        n = len(pairs_)
+        args = ([p[0] for p in pairs_], [p[3] for p in pairs_], [p[2] for p in pairs_], [p[5] for p in pairs_],
+                [S.host_strides(p[1]) for p in pairs_], [S.host_strides(p[4]) for p in pairs_])
+        with S.SSIMU2Batch(n, S.STORAGE_FP32) as b:
+            b.set_tuning(HANDLER_BLUR_CONFIG, 2, 2)
+            got32 = b.compute(*args)
+            f32 = b.features(n)
+            again = b.compute(*args)  # a replay with the same composition
+        with S.SSIMU2Batch(n, S.STORAGE_FP16) as b:
+            got16 = b.compute(*args)
+        for i, (c1, r, _, c2, d, _) in enumerate(pairs_):
+            s1, s2 = S.host_strides(r), S.host_strides(d)
+            with S.SSIMU2Handler(c1, c2) as h:
+                plain = h.compute(r, d, s1, s2)
+                pf = h.features()
+            want, _ = oracle.ssimu2(c1, r, s1, c2, d, s2)
+            _say(f"resampled batch pair {i}: fp32 {got32[i]:.6f} plain {plain:.6f} fp16 {got16[i]:.6f} "
+                 f"oracle {want:.6f}")
+            np.testing.assert_array_equal(f32[i], pf)
+            assert abs(got32[i] - plain) <= 1e-12
+            assert abs(got32[i] - want) <= 0.005
+            # FP16 storage: 0.01 on natural scores; the remap's polynomial
+            # scales feature error with the score below -100 (pair 4, two
+            # unrelated images)
+            assert abs(got16[i] - want) <= 0.01 * max(1.0, abs(want) / 100.0)
+        np.testing.assert_array_equal(again, got32)
+    finally:
+        for b in bufs:
+            b.free()

@@ -377,8 +377,10 @@ def test_crop_and_resize_errors():
         d = S.DeviceBuffer(64 * 48 * 2)
         try:
             ptrs = [d.address, d.address + 64 * 48, d.address + 64 * 48 + 32 * 24]
+            # batches resample too (tests/test_gpu_batch.py); a pair whose
+            # final sizes differ is refused as by the handler
             with pytest.raises(S.VshipError) as e:
-                b.compute([other], [other], [ptrs], [ptrs], [[64, 32, 32]], [[64, 32, 32]])
-            assert e.value.code == 5
+                b.compute([cs], [other], [ptrs], [ptrs], [[64, 32, 32]], [[64, 32, 32]])
+            assert e.value.code == 4
         finally:
             d.free()

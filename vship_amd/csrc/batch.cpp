@@ -39,7 +39,8 @@ Ssimu2Batch::Ssimu2Batch(int max_pairs, bool fp16, int device)
     size_t off = 0;
     const size_t o_counts = off; off = align_up(off + kCountSlots * sizeof(int), 256);
     const size_t o_fjobs = off; off = align_up(off + cap_fjobs_ * sizeof(FrontJob), 256);
-    const size_t o_colors = off; off = align_up(off + 2 * cap_fjobs_ * sizeof(FrontColor), 256);
+    // colours: each image's own, and the linear-RGB input of a resampled image's pyramid job
+    const size_t o_colors = off; off = align_up(off + 4 * cap_fjobs_ * sizeof(FrontColor), 256);
     const size_t o_geoms = off; off = align_up(off + max_pairs * sizeof(PairGeom), 256);
     const size_t o_maps = off; off = align_up(off + cap_maps_ * sizeof(MapJob), 256);
     const size_t o_bmaps = off; off = align_up(off + cap_maps_ * sizeof(MapJob), 256);
@@ -77,6 +78,9 @@ Ssimu2Batch::~Ssimu2Batch()
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     (void)hipFree(xyb_);
+    (void)hipFree(rsz_);
+    (void)hipFree(d_rjobs_);
+    (void)hipHostFree(h_rjobs_);
     (void)hipFree(partials_);
     (void)hipFree(tables_);
     (void)hipFree(d_features_);
@@ -114,6 +118,10 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
     sjobs_.clear();
     xyb_off_.clear();
     pair_jobs_.clear();
+    resamples_.clear();
+    rjobs_.clear();
+    window_.assign(2 * (size_t)in.npairs, std::array<int64_t, 3>{-1, -1, -1});
+    rsz_elems_ = 0;
     exact_off_.assign(in.npairs, -1);
     const size_t elem = fp16_ ? 2 : 4;
     int64_t elems = 0;
@@ -128,9 +136,8 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
     for (int i = 0; i < in.npairs; i++) {
         const ImageSpec s1 = cached_spec(in.src[i]);
         const ImageSpec s2 = cached_spec(in.dis[i]);
-        if (s1.resample || s2.resample)
-            VSHIP_THROW(Vship_NonRGBSInput, "crop and resize are supported by the single-pair handler only");
-        if (s1.width != s2.width || s1.height != s2.height)
+        // the size the metric sees: after the resize and crop of a resampled image
+        if (s1.final_w != s2.final_w || s1.final_h != s2.final_h)
             VSHIP_THROW(Vship_DifferingInputType, "pair " + std::to_string(i) + ": sizes differ");
         const ImageSpec* specs[2] = {&s1, &s2};
         const uint8_t* const* pl[2] = {in.planes1 + 3 * i, in.planes2 + 3 * i};
@@ -148,7 +155,7 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
         const auto found = first_of.find(key);
         const int ref = found == first_of.end() ? -1 : found->second;
         if (ref < 0) first_of.emplace(std::move(key), i);
-        const PairLayout& L = cached_layout((int)s1.width, (int)s1.height, tuning_.blur_config);
+        const PairLayout& L = cached_layout((int)s1.final_w, (int)s1.final_h, tuning_.blur_config);
         geoms_.push_back(L.geom);
         FrontJob j;
         std::memset(&j, 0, sizeof(j));
@@ -178,9 +185,51 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
             }
             j.color[k] = (int32_t)colors_.size();
             colors_.push_back(specs[k]->fc);
+            if (specs[k]->resample && !(k == 0 && j.shared)) {
+                // crop/resize (reference vshipColor.hpp:87-103): linear RGB at
+                // the input size, resized; the pyramid job reads the cropped
+                // window of the resized planes as float linear RGB
+                const ImageSpec& sp = *specs[k];
+                Resample r;
+                r.width = sp.width;
+                r.height = sp.height;
+                r.target_w = sp.target_w;
+                r.target_h = sp.target_h;
+                r.lin = rsz_elems_;
+                rsz_elems_ += (int64_t)align_up((size_t)(3 * sp.width * sp.height), 64);
+                int64_t cur = r.lin;
+                r.horiz = r.vert = -1;
+                if (sp.target_w != sp.width) {
+                    r.horiz = cur = rsz_elems_;
+                    rsz_elems_ += (int64_t)align_up((size_t)(3 * sp.target_w * sp.height), 64);
+                }
+                if (sp.target_h != sp.height) {
+                    r.vert = cur = rsz_elems_;
+                    rsz_elems_ += (int64_t)align_up((size_t)(3 * sp.target_w * sp.target_h), 64);
+                }
+                FrontJob lj;
+                std::memset(&lj, 0, sizeof(lj));
+                for (int p = 0; p < 3; p++) {
+                    lj.plane[0][p] = pl[k][p];
+                    lj.stride[0][p] = ls[k][p];
+                }
+                lj.color[0] = j.color[k];
+                lj.width = (int32_t)sp.width;
+                lj.height = (int32_t)sp.height;
+                lj.tiles_x = (int32_t)((sp.width + kFrontTileW - 1) / kFrontTileW);
+                resamples_.push_back(r);
+                rjobs_.push_back(lj);
+                for (int p = 0; p < 3; p++) {
+                    window_[2 * (size_t)i + k][p] = cur + p * sp.target_w * sp.target_h + sp.crop_top * sp.target_w +
+                                                    sp.crop_left;
+                    j.stride[k][p] = sp.target_w * (int64_t)sizeof(float);
+                }
+                j.color[k] = (int32_t)colors_.size();
+                colors_.push_back(linear_rgb_color());
+            }
         }
-        j.width = (int32_t)s1.width;
-        j.height = (int32_t)s1.height;
+        j.width = (int32_t)s1.final_w;
+        j.height = (int32_t)s1.final_h;
         j.tiles_x = L.front_tiles_x;
         j.tile_start = L.front_tiles_per_image;  // tile count until the jobs are ordered
         j.pair = i;
@@ -271,9 +320,30 @@ void Ssimu2Batch::ensure_capacity()
         partials_cap_ = want;
         graph_valid_ = false;  // the kernels take the partials pointer as an argument
     }
+    if (rsz_elems_ > rsz_cap_) {
+        (void)hipFree(rsz_);
+        rsz_ = nullptr;
+        rsz_cap_ = 0;
+        const int64_t want = rsz_elems_ + rsz_elems_ / 8;
+        if (hipMalloc((void**)&rsz_, (size_t)want * sizeof(float)) != hipSuccess)
+            VSHIP_THROW(Vship_OutOfVRAM, "batch resample arena allocation failed");
+        rsz_cap_ = want;
+    }
+    if (!rjobs_.empty() && d_rjobs_ == nullptr) {
+        if (hipMalloc((void**)&d_rjobs_, 2 * (size_t)max_pairs_ * sizeof(FrontJob)) != hipSuccess)
+            VSHIP_THROW(Vship_OutOfVRAM, "batch resample job allocation failed");
+        if (hipHostMalloc((void**)&h_rjobs_, 2 * (size_t)max_pairs_ * sizeof(FrontJob), hipHostMallocDefault) !=
+            hipSuccess)
+            VSHIP_THROW(Vship_OutOfRAM, "batch pinned allocation failed");
+    }
     // final pointers into the arena, then one packed upload
     for (FrontJob& j : fjobs_) {
         for (int k = 0; k < 2; k++) j.out[k] = static_cast<uint8_t*>(xyb_) + xyb_off_[2 * j.pair + k] * elem;
+        for (int k = 0; k < 2; k++) {
+            const auto& w = window_[2 * (size_t)j.pair + k];
+            if (w[0] < 0 || (k == 0 && j.shared)) continue;
+            for (int p = 0; p < 3; p++) j.plane[k][p] = reinterpret_cast<const uint8_t*>(rsz_ + w[p]);
+        }
         const int64_t e = exact_off_[j.pair];
         j.exact0 = e < 0 ? nullptr : reinterpret_cast<float*>(static_cast<uint8_t*>(xyb_) + e * elem);
     }
@@ -357,6 +427,33 @@ void Ssimu2Batch::capture()
     graph_valid_ = true;
 }
 
+// Crop/resize of the resampled images (linear RGB at the input size, then
+// the horizontal and vertical resizes, as the single-pair handler does),
+// launched on the batch stream after the table upload and before the graph,
+// whose front-end reads the resized planes: one set of launches per
+// resampled image, outside the captured graph because their sizes vary
+// from batch to batch.
+void Ssimu2Batch::run_resamples()
+{
+    if (resamples_.empty()) return;
+    std::memcpy(h_rjobs_, rjobs_.data(), rjobs_.size() * sizeof(FrontJob));
+    VSHIP_HIP_CHECK(
+        hipMemcpyAsync(d_rjobs_, h_rjobs_, rjobs_.size() * sizeof(FrontJob), hipMemcpyHostToDevice, stream_));
+    const size_t lds = frontend_max_lds_bytes();
+    for (size_t r = 0; r < resamples_.size(); r++) {
+        const Resample& x = resamples_[r];
+        const int W = (int)x.width, H = (int)x.height, tw = (int)x.target_w, th = (int)x.target_h;
+        const int tiles = rjobs_[r].tiles_x * ((H + kFrontTileH - 1) / kFrontTileH);
+        VSHIP_HIP_CHECK(launch_linear_rgb(d_rjobs_ + r, tiles, d_colors_, rsz_ + x.lin, W, lds, stream_));
+        const float* cur = rsz_ + x.lin;
+        if (x.horiz >= 0) {
+            VSHIP_HIP_CHECK(launch_resize_h(rsz_ + x.horiz, cur, W, H, tw, stream_));
+            cur = rsz_ + x.horiz;
+        }
+        if (x.vert >= 0) VSHIP_HIP_CHECK(launch_resize_v(rsz_ + x.vert, cur, tw, H, th, stream_));
+    }
+}
+
 void Ssimu2Batch::compute(const BatchInput& in, double* scores, int scale_mask)
 {
     // features(), profile() and complete() refer to the last call that
@@ -368,6 +465,7 @@ void Ssimu2Batch::compute(const BatchInput& in, double* scores, int scale_mask)
     call_mask_ = scale_mask;
     build_tables(in);
     ensure_capacity();
+    run_resamples();
     if (!graph_valid_) capture();
     VSHIP_HIP_CHECK(hipGraphLaunch(exec_, stream_));
     VSHIP_HIP_CHECK(hipStreamSynchronize(stream_));
@@ -468,6 +566,7 @@ void Ssimu2Batch::autotune(const BatchInput& in, int reps, BatchTuning* chosen, 
         tuning_ = t;
         build_tables(in);
         ensure_capacity();
+        run_resamples();          // the front-end's input (not timed: the same for every tuning)
         launch_kernels(stream_);  // warm-up
         VSHIP_HIP_CHECK(hipEventRecord(e0, stream_));
         for (int r = 0; r < reps; r++) launch_kernels(stream_);

@@ -18,6 +18,7 @@
 #include <stdint.h>
 #include <vector>
 
+#include <array>
 #include <map>
 #include <string>
 #include <tuple>
@@ -113,10 +114,26 @@ private:
     int call_mask_ = kAllScales;  // scales the tables being built run
     std::vector<int> pair_jobs_;  // per pair: first index into maps_ (and count in maps_ order)
     double work_bytes_ = 0.0, work_flops_ = 0.0;
+    // crop/resize: per resampled image its linear-RGB job and its buffers in
+    // the resample arena (float offsets; the resized planes are the pyramid
+    // job's input, read from the cropped window)
+    struct Resample {
+        int64_t width, height, target_w, target_h;
+        int64_t lin, horiz, vert;    // float offsets of the linear, h-resized and resized planes (-1: none)
+    };
+    std::vector<Resample> resamples_;
+    std::vector<FrontJob> rjobs_;                // linear-RGB jobs, one per entry of resamples_
+    std::vector<std::array<int64_t, 3>> window_; // per image (2 per pair): float offset of each plane's window, -1
+    int64_t rsz_elems_ = 0;
+    void run_resamples();
 
     // device
     void* xyb_ = nullptr;
     int64_t xyb_cap_ = 0;            // elements
+    float* rsz_ = nullptr;           // resample arena
+    int64_t rsz_cap_ = 0;            // floats
+    FrontJob* d_rjobs_ = nullptr;    // linear-RGB jobs (2 per pair at most)
+    FrontJob* h_rjobs_ = nullptr;    // pinned staging
     float* partials_ = nullptr;
     int64_t partials_cap_ = 0;       // tiles
     uint8_t* tables_ = nullptr;      // packed tables
