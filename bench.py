@@ -621,7 +621,8 @@ def main():
         nproc = host_cores()
         rate1, vals1 = cpu_baseline(frames, 1, args.cpu_pairs_single)
         rate_n, vals_n = cpu_baseline(frames, WORKERS, args.cpu_pairs)
-        err = max(abs(a - scores[i % PAIRS]) for i, a in enumerate(vals_n + vals1))
+        # each sample is pairs 0, 1, ... of the step: compare pair i with the GPU's pair i
+        err = max(abs(a - scores[i % PAIRS]) for vals in (vals_n, vals1) for i, a in enumerate(vals))
         cpu = {"value": round(rate_n, 3), "unit": "pairs/s", "cores": WORKERS, "kind": "port",
                "sample": f"{args.cpu_pairs} of the 240 1080p pairs, FP64 oracle, one pair per thread, "
                          f"{WORKERS} threads ({nproc} cores visible; one GPU's share of the box is 16)",
