@@ -124,3 +124,21 @@ def out_of_gamut_pair(cs, seed: int):
             d = np.clip(d.astype(np.int32) + rng.integers(-6, 7, d.shape), 0, 255).astype(np.uint8)
         dist.append(d)
     return ref, dist, strides
+
+
+def wide_range_pair(w=192, h=130, gain=400.0, seed=3):
+    """Linear-light float RGB whose right half is `gain` times brighter (linear
+    values up to 300, XYB values up to about 7) and a copy perturbed by 2%
+    per sample: the blur's fixed first-attempt scale holds centred XYB
+    values up to 1.5 only, so items over the bright half and across the edge
+    are run again with a scale from their window's bound."""
+    cs = S.colorspace(w, h, sample=S.SAMPLE_FLOAT, rng=S.RANGE_FULL, subw=0, subh=0, family=S.FAMILY_RGB,
+                      matrix=S.MATRIX_RGB, trc=S.TRC_LINEAR)
+    rng = np.random.default_rng(seed)
+    ref, dist = [], []
+    for _ in range(3):
+        v = rng.uniform(0.25, 0.75, (h, w))
+        v[:, w // 2:] *= gain
+        ref.append(np.ascontiguousarray(v, dtype=np.float32))
+        dist.append(np.ascontiguousarray(v * (1.0 + rng.normal(0, 0.02, (h, w))), dtype=np.float32))
+    return cs, ref, dist, [w * 4] * 3

@@ -97,6 +97,40 @@ def test_fp16_batch_vs_oracle(pairs):
     assert worst <= 0.01, worst
 
 
+@pytest.mark.parametrize("storage,bar", [(S.STORAGE_FP32, 0.005), (S.STORAGE_FP16, 0.01)], ids=["fp32", "fp16"])
+def test_wide_range_input_in_batch(storage, bar):
+    """A pair whose bright half takes the blur's bounded-scale rerun, between
+    two ordinary pairs, in one batched call: all three within the storage's
+    bar of the FP64 oracle."""
+    wcs, wref, wdist, wst = cases.wide_range_pair()
+    cs = S.colorspace(131, 75)
+    ref, dist, st = cases.perturbed_pair(cs, seed=4)
+    host = [(cs, ref, dist, st), (wcs, wref, wdist, wst), (cs, dist, ref, st)]
+    bufs, dptr = [], []
+    for _, r, d, _st in host:
+        pair = []
+        for img in (r, d):
+            trip = []
+            for p in img:
+                b = S.DeviceBuffer(p.nbytes)
+                b.upload(np.ascontiguousarray(p))
+                bufs.append(b)
+                trip.append(b.address)
+            pair.append(trip)
+        dptr.append(pair)
+    try:
+        with S.SSIMU2Batch(4, storage) as b:
+            got = b.compute([h[0] for h in host], [h[0] for h in host], [p[0] for p in dptr],
+                            [p[1] for p in dptr], [h[3] for h in host], [h[3] for h in host])
+    finally:
+        for b in bufs:
+            b.free()
+    for i, (c, r, d, s) in enumerate(host):
+        want, _ = oracle.ssimu2(c, r, s, c, d, s)
+        _say("wide-range batch", i, got[i], want, got[i] - want)
+        assert abs(got[i] - want) <= bar
+
+
 def test_graph_replay_with_changing_compositions(pairs):
     with S.SSIMU2Batch(16, S.STORAGE_FP16) as b:
         first = b.compute(*pairs.args())

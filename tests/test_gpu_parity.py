@@ -61,6 +61,18 @@ def test_out_of_gamut_content():
     assert abs(got - want) <= 0.05
 
 
+def test_wide_range_input_vs_oracle():
+    cs, ref, dist, st = cases.wide_range_pair()
+    with S.SSIMU2Handler(cs, cs) as hd:
+        got = hd.compute(ref, dist, st, st)
+        feats = hd.features()
+        assert hd.compute(ref, ref, st, st) == 100.0
+    want, wf = _oracle_score(cs, ref, st, cs, dist, st)
+    _say("wide range", got, want, got - want)
+    assert abs(got - want) <= SCORE_TOL, (got, want)
+    np.testing.assert_allclose(feats, wf, rtol=2e-3, atol=2e-5)
+
+
 @pytest.mark.parametrize("w,h", [(1, 1), (3, 2), (8, 8), (17, 9), (64, 64), (65, 33), (135, 67), (300, 200)])
 def test_sizes_vs_oracle(w, h):
     cs = S.colorspace(w, h)

@@ -25,14 +25,15 @@
 // Precision: FP16 operands carry FP32 values as hi + lo (hi = fp16 of x
 // rounded toward zero, lo = fp16(x - hi)), the taps likewise (round to
 // nearest); each product is hi*hi + lo*hi + hi*lo with FP32 accumulation,
-// about 22 significant bits. Each item first finds the largest centred
-// magnitude in its whole window and scales every sample by one power of two
-// from it, so hi and lo stay in FP16's range for all four quantities and the
-// two vertical products of an output block share one accumulator.
+// about 22 significant bits. Every sample of an item is centred on the
+// item's first sample and scaled by one power of two (fixed, or from a bound
+// over the window when the fixed one would leave FP16's range), so hi and lo
+// stay in range for all four quantities and the two vertical products of an
+// output block share one accumulator.
 //
 // Identity: the blurred quantities are a, b, a*b and (a - b)^2. For a == b
 // every operation on a and b is the same and (a - b)^2 is 0, so the SSIM
-// ratio's numerator and denominator are the same float and the map is 0.
+// variance and the edge difference are 0 and so are the maps.
 #include "ssimu2_common.hpp"
 #include "color_device.hpp"
 #include "blur_maps.hpp"
@@ -59,6 +60,8 @@ constexpr int kVecPerRow = kWinC / 4;         // 4-sample vectors per window row
 constexpr int kVecPerBlock = 16 * kVecPerRow; // per image and block (320)
 constexpr int kStageIters = (kVecPerBlock + kBlurThreads - 1) / kBlurThreads;
 constexpr int kMaxSegb = 16;                  // items of at most 16 row blocks (256 rows)
+constexpr int kFirstScaleExp = 6;             // first attempt's scale 2^6 ...
+constexpr float kFirstScaleMax = 96.0f;       // ... and the largest |A|, |B| it accepts
 static_assert(kBlurThreads == 256, "four waves, one 16-column strip each");
 
 // Wave reductions on the DPP network: row shifts 1/2/4/8 reduce each 16-lane
@@ -130,12 +133,6 @@ __device__ inline void split4(const float (&x)[4], float one, half4& hi, half4& 
     lo[3] = l23[1];
 }
 
-__device__ inline void split_rn(float x, _Float16& hi, _Float16& lo)
-{
-    hi = (_Float16)x;
-    lo = (_Float16)(x - (float)hi);
-}
-
 __device__ inline floatx4 mma32(const half8& ah, const half8& al, const half8& bh, const half8& bl)
 {
     floatx4 c = {0.f, 0.f, 0.f, 0.f};
@@ -145,8 +142,8 @@ __device__ inline floatx4 mma32(const half8& ah, const half8& al, const half8& b
     return c;
 }
 
-// Toeplitz fragments of one lane (select chains: a lane-varying index into
-// the taps would put them in scratch memory).
+// Toeplitz fragments of one lane, looked up in an LDS copy of the host-split
+// band of FP16 tap pairs.
 //
 // Vertical pass: output block rb is Tfull * S with the 16x32 band
 // Tfull[m][r] = g[r - m] and S the 32 window rows of blocks rb and rb+1
@@ -164,32 +161,30 @@ struct Toeplitz {
     half8 tw_hi, tw_lo;  // the same with the two blocks' halves of k swapped
 };
 
-__device__ inline float tap_at(const BlurTaps& taps, int k)
+__device__ inline _Float16 half_bits(uint32_t v)
 {
-    float g = 0.0f;
-#pragma unroll
-    for (int m = 0; m < 17; m++) g = (k == m) ? taps.g[m] : g;
-    return g;
+    return __builtin_bit_cast(_Float16, (uint16_t)(v & 0xffffu));
 }
 
-__device__ inline Toeplitz make_toeplitz(const BlurTaps& taps, int lane)
+// band: the LDS copy of BlurTaps::band (g[k] as hi | lo << 16 at k + 16).
+__device__ inline Toeplitz make_toeplitz(const uint32_t* band, int lane)
 {
     Toeplitz T;
     const int ln = lane & 15, lg = lane >> 4;
 #pragma unroll
     for (int j = 0; j < 8; j++) {
-        _Float16 h, l;
-        split_rn(tap_at(taps, 8 * lg + j - ln), h, l);
-        T.g_hi[j] = h;
-        T.g_lo[j] = l;
+        // band offsets k - ln + 16 with k <= 31 and ln <= 15: all inside 1..47
+        const uint32_t gv = band[16 + 8 * lg + j - ln];
+        T.g_hi[j] = half_bits(gv);
+        T.g_lo[j] = half_bits(gv >> 16);
         const int r = j < 4 ? 4 * lg + j : 16 + 4 * lg + (j - 4);
-        split_rn(tap_at(taps, r - ln), h, l);
-        T.tv_hi[j] = h;
-        T.tv_lo[j] = l;
+        const uint32_t tv = band[16 + r - ln];
+        T.tv_hi[j] = half_bits(tv);
+        T.tv_lo[j] = half_bits(tv >> 16);
         const int rw = j < 4 ? 16 + 4 * lg + j : 4 * lg + (j - 4);
-        split_rn(tap_at(taps, rw - ln), h, l);
-        T.tw_hi[j] = h;
-        T.tw_lo[j] = l;
+        const uint32_t tw = band[16 + rw - ln];
+        T.tw_hi[j] = half_bits(tw);
+        T.tw_lo[j] = half_bits(tw >> 16);
     }
     return T;
 }
@@ -357,9 +352,11 @@ struct WindowBound<__half> {
 // Scaled centred values of a staged block into its ring slot: A = (a - c) s
 // and B = (b - c) s (0 for fill samples). s is a power of two, so
 // A = fma(a, s, -c s) is s * (a - c) rounded once. FP16 planes hold the
-// second image as d = b - a: B = A + d s.
+// second image as d = b - a: B = A + d s. mag collects the largest |A|, |B|
+// written (NaN samples are passed over, as by the window bound).
 template <typename StoreT>
-__device__ inline void stage_store(const StageRegs<StoreT>& r, int kb, float s, float cs, float* sA, float* sB)
+__device__ inline void stage_store(const StageRegs<StoreT>& r, int kb, float s, float cs, float* sA, float* sB,
+                                   float& mag)
 {
     const int t = threadIdx.x;
 #pragma unroll
@@ -375,6 +372,7 @@ __device__ inline void stage_store(const StageRegs<StoreT>& r, int kb, float s, 
         for (int i = 0; i < 4; i++) {
             va[i] = fmaf(av[i], s, -cs);
             vb[i] = (sizeof(StoreT) == 2) ? fmaf(bv[i], s, va[i]) : fmaf(bv[i], s, -cs);
+            mag = fmaxf(mag, fmaxf(fabsf(va[i]), fabsf(vb[i])));
         }
         const int off = (kb % kSlots) * kSlotFloats + row * kRP + 4 * v;
         *reinterpret_cast<float4*>(sA + off) = make_float4(va[0], va[1], va[2], va[3]);
@@ -396,11 +394,17 @@ __device__ inline void split_pair(float x0, float x1, float one, _Float16& h0, _
 }
 
 // One item. Every quantity is blurred in a scaled domain: the window values
-// are A = (a - c) s, B = (b - c) s with one power of two s for the whole item,
-// chosen so that |A|, |B| < 2^6 (so |A B| < 2^12 and (A - B)^2 < 2^14 stay in
-// FP16's range and hi/lo keep 22 significant bits; the subnormal floor of lo
-// is 2^-24 / s^2 of the item's magnitude, far below anything the maps
-// resolve). The blurred quantities are A, B, A B and (A - B)^2, and
+// are A = (a - c) s, B = (b - c) s with one power of two s for the whole item.
+// The first attempt (kExact false) takes s = 2^6, which keeps |A|, |B| <= 96
+// for any window whose centred XYB values stay within 1.5 (all SDR content:
+// its X', Y' and B' lie in about [0, 1.3]); the staging checks every value it
+// writes, and an item that exceeds 96 anywhere (HDR or float input, or
+// inf) reports it and is run again with kExact, whose s comes from a bound
+// over the whole window (|A|, |B| < 2^6). Either way |A B| <= 2^13.2 and
+// (A - B)^2 <= 2^15.2 < 65504 stay in FP16's range; hi/lo keep 22 significant
+// bits down to lo's subnormal floor of 2^-24 / s (2^-24 / s^2 for products),
+// absolute errors of 1e-9 and 1e-11 at s = 2^6, which the maps cannot resolve
+// against their constants C2 = 9e-4 and 1. The blurred quantities are A, B, A B and (A - B)^2, and
 // sigma1^2 + sigma2^2 = 2 sigma12 + var(A - B), var(A - B) =
 // E[(A - B)^2] - (mu1 - mu2)^2: for a == b it is 0 exactly and so is the map.
 // The maps are evaluated on the scaled
@@ -409,11 +413,11 @@ __device__ inline void split_pair(float x0, float x1, float one, _Float16& h0, _
 // ratio (|B - M2| + s) / (|A - M1| + s) equals (|b - mu2| + 1) / (|a - mu1| + 1),
 // so the maps are those of the unscaled values bit for bit. Outputs outside
 // the image have zero normalisers and zero samples, which makes both maps 0
-// exactly (the SSIM ratio is C2 s^2 / C2 s^2 = 1, the edge ratio s / s), so
-// no output is masked by a branch. Both edge maps are always summed (the
+// exactly (the SSIM variance and the edge difference are 0), so no output
+// is masked by a branch. Both edge maps are always summed (the
 // scorers read only the maps a job's flags select); NQ = 4 jobs add SSIM.
-template <typename StoreT, int NQ>
-__device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, const BlurTaps& taps,
+template <typename StoreT, int NQ, bool kExact>
+__device__ bool blur_item(const MapJob& job, int tile_x, int tile_y, int segb, const BlurTaps& taps,
                           const ToeplitzLds* TL, float* sA, float* sB, float* red, float* rowinv,
                           float* __restrict__ partial)
 {
@@ -447,43 +451,49 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
     // 1 / (in-bounds tap sum) of each output row of the item, 0 below the image
     if (t < 16 * segb) rowinv[t] = inv_norm(g.ty0 + t, g.h, taps);
 
-    // the item's scale from a bound over the whole window: two blocks per
-    // step so that their loads are in flight together, in reverse order so
-    // that block 0 is the one left in st
     StageRegs<StoreT> st;
-    {
-        WindowBound<StoreT> bnd(c);
-        StageRegs<StoreT> st2;
-        for (int kb = nwb - 1; kb > 0; kb -= 2) {
-            stage_load<StoreT>(job.a, job.b, g, kb, st2, fa, fb);
-            stage_load<StoreT>(job.a, job.b, g, kb - 1, st, fa, fb);
-            bnd.add(st2, c);
-            bnd.add(st, c);
+    int e1 = kFirstScaleExp;
+    if constexpr (kExact) {
+        // the item's scale from a bound over the whole window: two blocks per
+        // step so that their loads are in flight together, in reverse order so
+        // that block 0 is the one left in st
+        {
+            WindowBound<StoreT> bnd(c);
+            StageRegs<StoreT> st2;
+            for (int kb = nwb - 1; kb > 0; kb -= 2) {
+                stage_load<StoreT>(job.a, job.b, g, kb, st2, fa, fb);
+                stage_load<StoreT>(job.a, job.b, g, kb - 1, st, fa, fb);
+                bnd.add(st2, c);
+                bnd.add(st, c);
+            }
+            if (nwb & 1) {
+                stage_load<StoreT>(job.a, job.b, g, 0, st, fa, fb);
+                bnd.add(st, c);
+            }
+            const uint32_t mbits = wave_max_bits(__float_as_uint(bnd.value()));
+            uint32_t* redu = reinterpret_cast<uint32_t*>(red);
+            if (lane == 0) redu[wave] = mbits;
         }
-        if (nwb & 1) {
-            stage_load<StoreT>(job.a, job.b, g, 0, st, fa, fb);
-            bnd.add(st, c);
+        __syncthreads();
+        const uint32_t* redu = reinterpret_cast<const uint32_t*>(red);
+        const float vmax = __uint_as_float(max(max(redu[0], redu[1]), max(redu[2], redu[3])));
+        e1 = 6;
+        if (vmax > 0.0f) {
+            int ex;
+            frexpf(vmax, &ex);  // vmax < 2^ex
+            e1 = 6 - ex;
         }
-        const uint32_t mbits = wave_max_bits(__float_as_uint(bnd.value()));
-        uint32_t* redu = reinterpret_cast<uint32_t*>(red);
-        if (lane == 0) redu[wave] = mbits;
+        e1 = max(-60, min(e1, 60));
+    } else {
+        stage_load<StoreT>(job.a, job.b, g, 0, st, fa, fb);
     }
-    __syncthreads();
-    const uint32_t* redu = reinterpret_cast<const uint32_t*>(red);
-    const float vmax = __uint_as_float(max(max(redu[0], redu[1]), max(redu[2], redu[3])));
-    int e1 = 6;
-    if (vmax > 0.0f) {
-        int ex;
-        frexpf(vmax, &ex);  // vmax < 2^ex
-        e1 = 6 - ex;
-    }
-    e1 = max(-60, min(e1, 60));
     const float s = ldexpf(1.0f, e1);
     const float cs = c * s;
     const float inv_s2 = ldexpf(1.0f, -2 * e1);
     const float c2s2 = kC2 * ldexpf(1.0f, 2 * e1);
+    float mag = 0.0f;
 
-    stage_store<StoreT>(st, 0, s, cs, sA, sB);
+    stage_store<StoreT>(st, 0, s, cs, sA, sB, mag);
     __syncthreads();
 
     const int x = g.tx0 + 16 * wave + ln;
@@ -578,29 +588,32 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
             const float rnv[4] = {rn.x, rn.y, rn.z, rn.w};
 #pragma unroll
             for (int i = 0; i < 4; i++) {
+#pragma clang fp contract(off)
                 const float nrm = inx * rnv[i];
                 const float m1 = acc[0][i] * nrm, m2 = acc[1][i] * nrm;
                 if constexpr (NQ == 4) {
-#pragma clang fp contract(off)
-                    const float s12 = acc[2][i] * nrm;
-                    const float dsq = acc[3][i] * nrm;
                     // 1 - num_m num_s / den_s = u + v - u v with
                     // u = (mu1 - mu2)^2, v = var(A - B) / den_s and
                     // den_s = 2 sigma12 + C2 + var(A - B): formed from the
-                    // small quantities, not as a difference from 1
+                    // small quantities, not as a difference from 1. One
+                    // reciprocal (1 ulp): the map needs no more, and for
+                    // var = 0 (a == b) v is 0 whatever it rounds.
                     const float md = m1 - m2, md2 = md * md;
                     const float u = md2 * inv_s2;
-                    const float var = dsq - md2;
-                    const float num_s = 2.0f * (s12 - m1 * m2) + c2s2;
-                    const float v = div_nr(var, num_s + var);
+                    const float var = fmaf(acc[3][i], nrm, -md2);      // E[(A - B)^2] - (mu1 - mu2)^2
+                    const float cov = fmaf(-m1, m2, acc[2][i] * nrm);  // sigma12
+                    const float num_s = fmaf(2.0f, cov, c2s2);
+                    const float v = var * __builtin_amdgcn_rcpf(num_s + var);
                     const float val = fmaxf(fmaf(-u, v, u + v), 0.0f);
                     const float v2 = val * val;
                     s1m[0] += val;
                     s4m[0] = fmaf(v2, v2, s4m[0]);
                 }
                 {
-#pragma clang fp contract(off)
-                    const float dd = div_nr(fabsf(cb[i * kRP] - m2) + s, fabsf(ca[i * kRP] - m1) + s) - 1.0f;
+                    // (|B - M2| + s) / (|A - M1| + s) - 1 as (|B - M2| - |A - M1|) / (|A - M1| + s):
+                    // the same ratio without the cancellation, and exactly 0 when a == b
+                    const float da = fabsf(ca[i * kRP] - m1), db = fabsf(cb[i * kRP] - m2);
+                    const float dd = (db - da) * __builtin_amdgcn_rcpf(da + s);
                     const float art = fmaxf(dd, 0.0f), a2 = art * art;
                     const float det = fmaxf(-dd, 0.0f), d2 = det * det;
                     s1m[1] += art;
@@ -610,7 +623,7 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
                 }
             }
         }
-        if (more) stage_store<StoreT>(st, kb + 1, s, cs, sA, sB);
+        if (more) stage_store<StoreT>(st, kb + 1, s, cs, sA, sB, mag);
         __syncthreads();
     };
     for (int kb = 0; kb < nwb; kb += 2) {
@@ -618,15 +631,21 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
         if (kb + 1 < nwb) step(std::integral_constant<int, 1>{}, kb + 1);
     }
 
-    // ---- workgroup reduction of the 6 sums
+    // ---- workgroup reduction of the 6 sums (and of the first attempt's range check)
     float v6[6] = {s1m[0], s1m[1], s1m[2], s4m[0], s4m[1], s4m[2]};
 #pragma unroll
     for (int m = 0; m < 6; m++) v6[m] = wave_sum(v6[m]);
+    bool over = false;
+    if constexpr (!kExact) over = __builtin_amdgcn_ballot_w64(!(mag <= kFirstScaleMax)) != 0;
     if (lane == 0) {
 #pragma unroll
         for (int m = 0; m < 6; m++) red[wave * 8 + m] = v6[m];
+        red[wave * 8 + 6] = over ? 1.0f : 0.0f;
     }
     __syncthreads();
+    if constexpr (!kExact) {
+        if (red[6] + red[14] + red[22] + red[30] != 0.0f) return false;  // the same for every thread
+    }
     if (t < 6) {
         float tot = red[t];
         tot += red[8 + t];
@@ -634,6 +653,7 @@ __device__ void blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
         tot += red[24 + t];
         partial[t] = tot;
     }
+    return true;
 }
 
 template <typename StoreT>
@@ -649,6 +669,7 @@ void blur_strip_kernel(const MapJob* __restrict__ jobs, const int* __restrict__ 
     __shared__ float red[32];
     __shared__ __attribute__((aligned(16))) float rowinv[16 * kMaxSegb];
     __shared__ ToeplitzLds toe;
+    __shared__ uint32_t band[48];
     const int total = counts[kCountBlurTiles];
     const int njobs = counts[kCountBlurJobs];
     const int per = (total + kXcds - 1) / kXcds;
@@ -657,7 +678,9 @@ void blur_strip_kernel(const MapJob* __restrict__ jobs, const int* __restrict__ 
     const int end = min((xcd + 1) * per, total);
     const int first = xcd * per + blockIdx.x / kXcds;
     if (first >= end) return;  // a grid sized for larger batches: nothing to set up
-    if (threadIdx.x < 64) toeplitz_to_lds(make_toeplitz(taps, threadIdx.x), threadIdx.x, &toe);
+    if (threadIdx.x < 48) band[threadIdx.x] = taps.band[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x < 64) toeplitz_to_lds(make_toeplitz(band, threadIdx.x), threadIdx.x, &toe);
     __syncthreads();
     float* sA = ring;
     float* sB = ring + kSlots * kSlotFloats;
@@ -672,12 +695,18 @@ void blur_strip_kernel(const MapJob* __restrict__ jobs, const int* __restrict__ 
         const int tile_x = local % job.tiles_x;
         const int tile_y = local / job.tiles_x;
         __syncthreads();  // LDS of the previous item is free
-        if (job.flags & kMapSsim)
-            blur_item<StoreT, 4>(job, tile_x, tile_y, segb, taps, &toe, sA, sB, red, rowinv,
-                                 partials + (int64_t)(job.part_start + local) * kPartialStride);
-        else
-            blur_item<StoreT, 2>(job, tile_x, tile_y, segb, taps, &toe, sA, sB, red, rowinv,
-                                 partials + (int64_t)(job.part_start + local) * kPartialStride);
+        float* part = partials + (int64_t)(job.part_start + local) * kPartialStride;
+        if (job.flags & kMapSsim) {
+            if (!blur_item<StoreT, 4, false>(job, tile_x, tile_y, segb, taps, &toe, sA, sB, red, rowinv, part)) {
+                __syncthreads();
+                blur_item<StoreT, 4, true>(job, tile_x, tile_y, segb, taps, &toe, sA, sB, red, rowinv, part);
+            }
+        } else {
+            if (!blur_item<StoreT, 2, false>(job, tile_x, tile_y, segb, taps, &toe, sA, sB, red, rowinv, part)) {
+                __syncthreads();
+                blur_item<StoreT, 2, true>(job, tile_x, tile_y, segb, taps, &toe, sA, sB, red, rowinv, part);
+            }
+        }
     }
 }
 

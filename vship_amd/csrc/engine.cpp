@@ -304,6 +304,15 @@ BlurTaps make_blur_taps()
     }
     // the same float expression as the kernels' border case with beg = 0, end = 17
     t.inv_full = 1.0f / (t.integral[17] - t.integral[0]);
+    for (int k = -16; k < 32; k++) {
+        uint32_t v = 0;
+        if (k >= 0 && k < 17) {
+            const _Float16 hi = (_Float16)t.g[k];  // IEEE round to nearest, as on the device
+            const _Float16 lo = (_Float16)(t.g[k] - (float)hi);
+            v = (uint32_t)__builtin_bit_cast(uint16_t, hi) | ((uint32_t)__builtin_bit_cast(uint16_t, lo) << 16);
+        }
+        t.band[k + 16] = v;
+    }
     return t;
 }
 
