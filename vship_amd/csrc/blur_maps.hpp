@@ -20,6 +20,18 @@ __device__ inline float inv_norm(int x, int w, const BlurTaps& taps)
     return 1.0f / (taps.integral[end] - taps.integral[beg]);
 }
 
+// The same renormalisation for blurs with the normalised taps (BlurTaps::band,
+// which sum to 1): (sum of all 17 taps) / (sum of the in-bounds taps), exactly
+// 1 where all taps are in bounds, 0 outside the axis.
+__device__ inline float rel_norm(int x, int w, const BlurTaps& taps)
+{
+    if (x >= kHalo && x + kHalo < w) return 1.0f;
+    if (x >= w) return 0.0f;
+    const int beg = x < kHalo ? kHalo - x : 0;
+    const int end = (w - x + kHalo) < 17 ? (w - x + kHalo) : 17;
+    return (taps.integral[17] - taps.integral[0]) / (taps.integral[end] - taps.integral[beg]);
+}
+
 // n / d from the hardware reciprocal and one Newton correction: within an
 // ulp of IEEE division, and exactly 1 when n == d (the residual n - d * q is
 // exact and the correction lands on 1), which keeps identical inputs at a

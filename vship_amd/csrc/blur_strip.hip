@@ -449,7 +449,7 @@ __device__ bool blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
     const int nwb = nrb + 1;
 
     // 1 / (in-bounds tap sum) of each output row of the item, 0 below the image
-    if (t < 16 * segb) rowinv[t] = inv_norm(g.ty0 + t, g.h, taps);
+    if (t < 16 * segb) rowinv[t] = rel_norm(g.ty0 + t, g.h, taps);
 
     StageRegs<StoreT> st;
     int e1 = kFirstScaleExp;
@@ -497,7 +497,9 @@ __device__ bool blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
     __syncthreads();
 
     const int x = g.tx0 + 16 * wave + ln;
-    const float inx = inv_norm(x, g.w, taps);  // 0 at or beyond w
+    const float inx = rel_norm(x, g.w, taps);  // 0 at or beyond w
+    // the wave's 16 columns all have their 17 taps in bounds
+    const bool cols_inside = g.tx0 + 16 * wave >= kHalo && g.tx0 + 16 * wave + 15 + kHalo < g.w;
     float one = 1.0f;
     asm volatile("" : "+s"(one));  // emits nothing: keeps fma(x, one, -hi) an FMA
     float s1m[3] = {0.f, 0.f, 0.f}, s4m[3] = {0.f, 0.f, 0.f};
@@ -584,44 +586,68 @@ __device__ bool blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
                 ca = sA + slot * kSlotFloats + rr * kRP + 16 * wave + kHalo + ln;
                 cb = sB + slot * kSlotFloats + rr * kRP + 16 * wave + kHalo + ln;
             }
-            const float4 rn = *reinterpret_cast<const float4*>(rowinv + 16 * rbk + 4 * lg);
-            const float rnv[4] = {rn.x, rn.y, rn.z, rn.w};
+            // the maps of the lane's 4 outputs; with every tap of the output
+            // block in bounds (the wave's columns and the block's 16 rows) the
+            // products are the means as they stand (the band sums to 1),
+            // elsewhere they take the relative renormalisation
+            auto maps = [&](auto inside) {
+                constexpr bool kInside = decltype(inside)::value;
+                float rnv[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+                if constexpr (!kInside) {
+                    const float4 rn = *reinterpret_cast<const float4*>(rowinv + 16 * rbk + 4 * lg);
+                    rnv[0] = rn.x; rnv[1] = rn.y; rnv[2] = rn.z; rnv[3] = rn.w;
+                }
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
+                for (int i = 0; i < 4; i++) {
 #pragma clang fp contract(off)
-                const float nrm = inx * rnv[i];
-                const float m1 = acc[0][i] * nrm, m2 = acc[1][i] * nrm;
-                if constexpr (NQ == 4) {
-                    // 1 - num_m num_s / den_s = u + v - u v with
-                    // u = (mu1 - mu2)^2, v = var(A - B) / den_s and
-                    // den_s = 2 sigma12 + C2 + var(A - B): formed from the
-                    // small quantities, not as a difference from 1. One
-                    // reciprocal (1 ulp): the map needs no more, and for
-                    // var = 0 (a == b) v is 0 whatever it rounds.
-                    const float md = m1 - m2, md2 = md * md;
-                    const float u = md2 * inv_s2;
-                    const float var = fmaf(acc[3][i], nrm, -md2);      // E[(A - B)^2] - (mu1 - mu2)^2
-                    const float cov = fmaf(-m1, m2, acc[2][i] * nrm);  // sigma12
-                    const float num_s = fmaf(2.0f, cov, c2s2);
-                    const float v = var * __builtin_amdgcn_rcpf(num_s + var);
-                    const float val = fmaxf(fmaf(-u, v, u + v), 0.0f);
-                    const float v2 = val * val;
-                    s1m[0] += val;
-                    s4m[0] = fmaf(v2, v2, s4m[0]);
+                    float m1, m2, s12, dsq;
+                    if constexpr (kInside) {
+                        m1 = acc[0][i];
+                        m2 = acc[1][i];
+                        s12 = acc[2 % NQ][i];  // NQ = 2 jobs have no SSIM quantities
+                        dsq = acc[3 % NQ][i];
+                    } else {
+                        const float nrm = inx * rnv[i];
+                        m1 = acc[0][i] * nrm;
+                        m2 = acc[1][i] * nrm;
+                        s12 = acc[2 % NQ][i] * nrm;
+                        dsq = acc[3 % NQ][i] * nrm;
+                    }
+                    if constexpr (NQ == 4) {
+                        // 1 - num_m num_s / den_s = u + v - u v with
+                        // u = (mu1 - mu2)^2, v = var(A - B) / den_s and
+                        // den_s = 2 sigma12 + C2 + var(A - B): formed from the
+                        // small quantities, not as a difference from 1. One
+                        // reciprocal (1 ulp): the map needs no more, and for
+                        // var = 0 (a == b) v is 0 whatever it rounds.
+                        const float md = m1 - m2, md2 = md * md;
+                        const float u = md2 * inv_s2;
+                        const float var = dsq - md2;              // E[(A - B)^2] - (mu1 - mu2)^2
+                        const float cov = fmaf(-m1, m2, s12);     // sigma12
+                        const float num_s = fmaf(2.0f, cov, c2s2);
+                        const float v = var * __builtin_amdgcn_rcpf(num_s + var);
+                        const float val = fmaxf(fmaf(-u, v, u + v), 0.0f);
+                        const float v2 = val * val;
+                        s1m[0] += val;
+                        s4m[0] = fmaf(v2, v2, s4m[0]);
+                    }
+                    {
+                        // (|B - M2| + s) / (|A - M1| + s) - 1 as (|B - M2| - |A - M1|) / (|A - M1| + s):
+                        // the same ratio without the cancellation, and exactly 0 when a == b
+                        const float da = fabsf(ca[i * kRP] - m1), db = fabsf(cb[i * kRP] - m2);
+                        const float dd = (db - da) * __builtin_amdgcn_rcpf(da + s);
+                        const float art = fmaxf(dd, 0.0f), a2 = art * art;
+                        const float det = fmaxf(-dd, 0.0f), d2 = det * det;
+                        s1m[1] += art;
+                        s4m[1] = fmaf(a2, a2, s4m[1]);
+                        s1m[2] += det;
+                        s4m[2] = fmaf(d2, d2, s4m[2]);
+                    }
                 }
-                {
-                    // (|B - M2| + s) / (|A - M1| + s) - 1 as (|B - M2| - |A - M1|) / (|A - M1| + s):
-                    // the same ratio without the cancellation, and exactly 0 when a == b
-                    const float da = fabsf(ca[i * kRP] - m1), db = fabsf(cb[i * kRP] - m2);
-                    const float dd = (db - da) * __builtin_amdgcn_rcpf(da + s);
-                    const float art = fmaxf(dd, 0.0f), a2 = art * art;
-                    const float det = fmaxf(-dd, 0.0f), d2 = det * det;
-                    s1m[1] += art;
-                    s4m[1] = fmaf(a2, a2, s4m[1]);
-                    s1m[2] += det;
-                    s4m[2] = fmaf(d2, d2, s4m[2]);
-                }
-            }
+            };
+            const int oy = g.ty0 + 16 * rbk;  // first output row of the block
+            if (cols_inside && oy >= kHalo && oy + 15 + kHalo < g.h) maps(std::true_type{});
+            else maps(std::false_type{});
         }
         if (more) stage_store<StoreT>(st, kb + 1, s, cs, sA, sB, mag);
         __syncthreads();

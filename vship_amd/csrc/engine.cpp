@@ -307,8 +307,9 @@ BlurTaps make_blur_taps()
     for (int k = -16; k < 32; k++) {
         uint32_t v = 0;
         if (k >= 0 && k < 17) {
-            const _Float16 hi = (_Float16)t.g[k];  // IEEE round to nearest, as on the device
-            const _Float16 lo = (_Float16)(t.g[k] - (float)hi);
+            const float gn = t.g[k] * t.inv_full;  // normalised: the band sums to 1
+            const _Float16 hi = (_Float16)gn;      // IEEE round to nearest, as on the device
+            const _Float16 lo = (_Float16)(gn - (float)hi);
             v = (uint32_t)__builtin_bit_cast(uint16_t, hi) | ((uint32_t)__builtin_bit_cast(uint16_t, lo) << 16);
         }
         t.band[k + 16] = v;

@@ -160,10 +160,11 @@ struct BlurTaps {
     float g[17];
     float integral[18];
     float inv_full;  // 1 / (integral[17] - integral[0]): all 17 taps in bounds
-    // The taps as FP16 hi + lo pairs for the matrix-core blur: entry k + 16
-    // (k = -16..31) holds hi | lo << 16 of g[k] (hi rounded to nearest,
-    // lo = fp16(g - hi)), 0 for k outside 0..16, so a Toeplitz band element
-    // g[i - j] is one lookup.
+    // The normalised taps g[k] * inv_full (they sum to 1, so the blur of an
+    // output whose 17 taps are all in bounds needs no renormalisation) as
+    // FP16 hi + lo pairs for the matrix-core blur: entry k + 16 (k = -16..31)
+    // holds hi | lo << 16 (hi rounded to nearest, lo = fp16(g - hi)), 0 for k
+    // outside 0..16, so a Toeplitz band element is one lookup.
     uint32_t band[48];
 };
 
