@@ -133,12 +133,14 @@ __device__ inline void eotf_hlg(float& r, float& g, float& b)
     r = v[0] * k; g = v[1] * k; b = v[2] * k;
 }
 
-// 3x3 row-major matrix times (x, y, z), in place.
+// 3x3 row-major matrix times (x, y, z), in place; each row as the chain
+// fma(m2, z, fma(m1, y, m0 x)), so that a row with unit m0 and zero terms is
+// the same float as its chain without them.
 __device__ inline void mat3_apply(const float m[9], float& x, float& y, float& z)
 {
-    const float a = fmaf(m[0], x, fmaf(m[1], y, m[2] * z));
-    const float b = fmaf(m[3], x, fmaf(m[4], y, m[5] * z));
-    const float c = fmaf(m[6], x, fmaf(m[7], y, m[8] * z));
+    const float a = fmaf(m[2], z, fmaf(m[1], y, m[0] * x));
+    const float b = fmaf(m[5], z, fmaf(m[4], y, m[3] * x));
+    const float c = fmaf(m[8], z, fmaf(m[7], y, m[6] * x));
     x = a; y = b; z = c;
 }
 

@@ -21,6 +21,8 @@
 #include "color_device.hpp"
 #include "kernels.hpp"
 
+#include <type_traits>
+
 #define VSHIP_INLINE __device__ __attribute__((always_inline)) inline
 
 namespace vship {
@@ -460,21 +462,23 @@ VSHIP_INLINE float dot5(const float w[5], const float v[5])
 VSHIP_INLINE void chroma_420_left(const FrontColor& fc, const float* s_chroma, float* s_mid, float P[3][kBH][kBW])
 {
     constexpr int ccols = (kFrontTileW >> 1) + 4, crows = (kFrontTileH >> 1) + 4;
-    constexpr int plane_elems = ccols * crows;
     constexpr int kRowStep = kFrontThreads / 32;
     constexpr int kIters = (2 * crows + kRowStep - 1) / kRowStep;
     const int t = threadIdx.x;
     const int tx = t % kTX, ty = t / kTX;
     const int m = t & 31;
-#pragma unroll 1
+    // staged row e = p * crows + r of plane p is row e of both the staged
+    // planes and s_mid (planes back to back), so item it of the thread reads
+    // and writes at fixed offsets from its first one
+    const float* src0 = s_chroma + (t >> 5) * ccols + m;
+    float* dst0 = s_mid + (t >> 5) * kMidPitch + 2 * m;
+#pragma unroll
     for (int it = 0; it < kIters; it++) {
-        const int e = (t >> 5) + kRowStep * it;
-        if (e >= 2 * crows) break;
-        const int p = e >= crows ? 1 : 0, r = e - p * crows;
-        const float* src = s_chroma + p * plane_elems + r * ccols + m;
+        if ((t >> 5) + kRowStep * it >= 2 * crows) break;
+        const float* src = src0 + kRowStep * it * ccols;
         const float v1 = src[1], v2 = src[2], v3 = src[3], v4 = src[4];
         const float h1 = fmaf(fc.hw[1][1], v1, fmaf(fc.hw[1][2], v2, fmaf(fc.hw[1][3], v3, fc.hw[1][4] * v4)));
-        *reinterpret_cast<float2*>(s_mid + (p * crows + r) * kMidPitch + 2 * m) = make_float2(v2, h1);
+        *reinterpret_cast<float2*>(dst0 + kRowStep * it * kMidPitch) = make_float2(v2, h1);
     }
     __syncthreads();
 #pragma unroll
@@ -714,18 +718,16 @@ VSHIP_INLINE void to_linear(const FrontJob& job, int k, const FrontColor& fc, in
     if constexpr (SPEC != kFrontGeneric) {
         // YCbCr matrix with unit luma column and R(Cb) = B(Cr) = 0 (front_spec
         // checks): mat3_apply's chains without the terms fmaf(0, v, t) == t
-        // and fmaf(1, Y, t) == Y + t, so the same floats (no contraction of
-        // the product into the sum)
-#pragma clang fp contract(off)
+        // and 1 * Y == Y, so the same floats
         const float* m = fc.pre;
 #pragma unroll
         for (int j = 0; j < kBH; j++)
 #pragma unroll
             for (int i = 0; i < kBW; i++) {
                 const float y = P[0][j][i], u = P[1][j][i], v = P[2][j][i];
-                P[0][j][i] = y + m[2] * v;
-                P[1][j][i] = y + fmaf(m[4], u, m[5] * v);
-                P[2][j][i] = y + m[7] * u;
+                P[0][j][i] = fmaf(m[2], v, y);
+                P[1][j][i] = fmaf(m[5], v, fmaf(m[4], u, y));
+                P[2][j][i] = fmaf(m[7], u, y);
             }
     } else if (fc.use_pre) {
 #pragma unroll
@@ -794,24 +796,32 @@ VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc,
 #pragma unroll
         for (int i = 0; i < kBW; i++) res.y0[j][i] = to_xyb(P[0][j][i], P[1][j][i], P[2][j][i], false).y;
 
-    // ---- level 1 (kBH/2 rows of one column per thread), edge clamp min(2x+1, w-1)
+    // ---- level 1 (kBH/2 rows of one column per thread), edge clamp min(2x+1, w-1);
+    // a tile inside the image needs no clamp (the same sums)
     float L1[kBH / 2][3];
-    const bool xlast = !(x0 + 1 < W);
+    auto level1 = [&](auto clamped) {
+        const bool xlast = decltype(clamped)::value && !(x0 + 1 < W);
+#pragma unroll
+        for (int j = 0; j < kBH / 2; j++) {
+            const bool ylast = decltype(clamped)::value && !(y0 + 2 * j + 1 < H);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float p00 = P[c][2 * j][0];
+                const float p10 = ylast ? p00 : P[c][2 * j + 1][0];
+                const float p01 = xlast ? p00 : P[c][2 * j][1];
+                const float p11 = ylast ? p01 : (xlast ? p10 : P[c][2 * j + 1][1]);
+                float s = p00;
+                s += p10;
+                s += p01;
+                s += p11;
+                L1[j][c] = s * 0.25f;
+            }
+        }
+    };
+    if ((tile_x + 1) * kFrontTileW <= W && (tile_y + 1) * kFrontTileH <= H) level1(std::false_type{});
+    else level1(std::true_type{});
 #pragma unroll
     for (int j = 0; j < kBH / 2; j++) {
-        const bool ylast = !(y0 + 2 * j + 1 < H);
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const float p00 = P[c][2 * j][0];
-            const float p10 = ylast ? p00 : P[c][2 * j + 1][0];
-            const float p01 = xlast ? p00 : P[c][2 * j][1];
-            const float p11 = ylast ? p01 : (xlast ? p10 : P[c][2 * j + 1][1]);
-            float s = p00;
-            s += p10;
-            s += p01;
-            s += p11;
-            L1[j][c] = s * 0.25f;
-        }
         const XYB q = to_xyb(L1[j][0], L1[j][1], L1[j][2], false);
         res.x1[j] = q.x;
         res.y1[j] = q.y;
