@@ -683,11 +683,12 @@ __device__ bool blur_item(const MapJob& job, int tile_x, int tile_y, int segb, c
 }
 
 template <typename StoreT>
-// 38.6 KB of LDS per workgroup. FP16 planes: 4 waves per SIMD, 128 registers
-// per lane (measured faster than 3 waves without the few spills 128 cost).
-// FP32 planes stage twice the registers and would spill inside the block loop
-// at 128: 3 waves per SIMD, no spills (measured 19% faster per pair).
-__global__ __launch_bounds__(kBlurThreads, sizeof(StoreT) == 2 ? 4 : 3)
+// 38.6 KB of LDS per workgroup; 3 waves per SIMD (168 registers per lane), no
+// spills for either storage. FP32 planes stage twice the registers and would
+// spill inside the block loop at 4 waves per SIMD (measured 19% slower per
+// pair); FP16 planes run as fast at 4 waves with 23 registers spilled at item
+// start (6.8 MB of scratch writes per 1080p pair), so they take 3 as well.
+__global__ __launch_bounds__(kBlurThreads, 3)
 void blur_strip_kernel(const MapJob* __restrict__ jobs, const int* __restrict__ counts, BlurTaps taps, int segb,
                        float* __restrict__ partials)
 {
