@@ -349,6 +349,20 @@ def valu_roof(counters, workload: str, kernel: str, ms: float, pairs: float):
             "mfma_busy_profiled": round(k.get("mfma_busy", float("nan")), 4)}
 
 
+def path_valu(counters, workload: str, kernels, pairs_per_s: float):
+    """VALU issue rate of the whole path over its timed region: the counted
+    VALU instructions of a pair's kernels times the pairs/s, against the peak
+    (a single pair's launch does not fill the GPU; the handlers' kernels
+    together do)."""
+    ks = counters.get(workload, {}).get("kernels", {})
+    if not all(k in ks and "valu_insts_per_pair" in ks[k] for k in kernels):
+        return None
+    lane_ops = sum(ks[k]["valu_insts_per_pair"] for k in kernels) * 64.0
+    achieved = lane_ops * pairs_per_s
+    return {"kernels": list(kernels), "lane_ops_per_pair": lane_ops, "achieved_tops": round(achieved / 1e12, 3),
+            "peak_tops": round(VALU_LANE_OPS_PEAK / 1e12, 3), "frac": round(achieved / VALU_LANE_OPS_PEAK, 4)}
+
+
 def run_plain_timed(handlers, call_args, scores, steps: int, warmup: int):
     for _ in range(warmup):
         run_plain(handlers, call_args, scores)
@@ -561,6 +575,7 @@ def main():
         "hbm": {"achieved": d["hbm_achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["hbm_frac"],
                 "algorithmic_bytes": d["algorithmic_bytes"]},
         "per_kernel": per_kernel, "path_bytes_per_pair": bytes_pair,
+        "path_valu": path_valu(counters, plain_wl, (FRONT_KERNEL, BLUR_KERNEL), value),
         "counters": os.path.relpath(COUNTERS_FILE, ROOT) if counters else None,
     }
 
@@ -598,6 +613,8 @@ def main():
             "blur_hbm_frac": round(blur_bytes16 / (b_blur * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "blur_valu": valu_roof(counters, batch_wl, "blur_strip_kernel<__half>", b_blur, PAIRS),
             "frontend_valu": valu_roof(counters, batch_wl, "frontend_kernel<__half, 1, 0>", b_front, PAIRS),
+            "path_valu": path_valu(counters, batch_wl, ("frontend_kernel<__half, 1, 0>", "blur_strip_kernel<__half>"),
+                                   b_value),
             "blur_fir_flops_tflops": round(b_flops / (b_blur * 1e-3) / 1e12, 2),
             "path_gbs": round(b_bytes / (b_graph * 1e-3) / 1e9, 1),
         }
