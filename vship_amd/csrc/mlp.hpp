@@ -1,10 +1,10 @@
 // mlp.hpp - the score surrogate: a 55 -> 128 -> 128 -> (mean, log-variance)
 // GELU network with BF16 weights and FP32 accumulation, trained and run by
-// the kernels in mlp.hip.
+// the kernels in mlp.hip (its layer products on the BF16 matrix cores).
 //
-// Inputs: the 54 SSIMULACRA2 features of scales 0-2 (plane * 18 + scale * 6 +
-// {ssim L1, artifact L1, detail L1, ssim L4, artifact L4, detail L4}) and the
-// natural log of the pixel count. Output: the predicted score and its
+// Inputs: the 54 SSIMULACRA2 features of three consecutive scales (0-2 or
+// 2-4; plane * 18 + scale * 6 + {ssim L1, artifact L1, detail L1, ssim L4,
+// artifact L4, detail L4}) and the natural log of the pixel count. Output: the predicted score and its
 // standard deviation. Inputs and target are normalised with per-column mean
 // and standard deviation fixed before training.
 //
@@ -29,6 +29,7 @@ constexpr int kMlpOffW3 = kMlpOffB2 + kMlpHidden;
 constexpr int kMlpOffB3 = kMlpOffW3 + kMlpOutputs * kMlpHidden;
 constexpr int kMlpParams = kMlpOffB3 + kMlpOutputs;  // 23938
 constexpr int kMlpSamplesPerGroup = 32;
+constexpr int kMlpInputsPadded = 64;  // row pitch of the BF16 W1 copy (inputs 55..63 are zero)
 constexpr int kMlpThreads = 256;
 
 // Device state passed by value to the kernels.
@@ -36,11 +37,11 @@ struct MlpDevice {
     float* master;     // kMlpParams FP32 weights
     float* adam_m;     // Adam first moment
     float* adam_v;     // Adam second moment
-    uint16_t* w1t;     // BF16 W1^T [55][128]
-    uint16_t* w2t;     // BF16 W2^T [128][128]
-    uint16_t* w3t;     // BF16 W3^T [128][2]
-    uint16_t* w2;      // BF16 W2 [128][128]
+    uint16_t* w1;      // BF16 W1 [128][kMlpInputsPadded], zero past input 55
+    uint16_t* w2;      // BF16 W2 [128][128] (forward products)
+    uint16_t* w2t;     // BF16 W2^T [128][128] (input gradients of layer 2)
     uint16_t* w3;      // BF16 W3 [2][128]
+    uint16_t* w3t;     // BF16 W3^T [128][2]
     const float* xmean;  // kMlpInputs
     const float* xinv;   // 1 / standard deviation (0 for constant columns)
     float ymean, ystd, yinv;

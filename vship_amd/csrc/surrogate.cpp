@@ -27,7 +27,9 @@ Surrogate::Surrogate(int max_batch, int device) : device_(device), max_batch_(ma
     dmalloc((void**)&dev_.master, kMlpParams * sizeof(float));
     dmalloc((void**)&dev_.adam_m, kMlpParams * sizeof(float));
     dmalloc((void**)&dev_.adam_v, kMlpParams * sizeof(float));
-    dmalloc((void**)&dev_.w1t, kMlpHidden * kMlpInputs * sizeof(uint16_t));
+    dmalloc((void**)&dev_.w1, kMlpHidden * kMlpInputsPadded * sizeof(uint16_t));
+    // the padding inputs 55..63 of the BF16 W1 copy stay zero (requantisation writes the rest)
+    VSHIP_HIP_CHECK(hipMemset(dev_.w1, 0, kMlpHidden * kMlpInputsPadded * sizeof(uint16_t)));
     dmalloc((void**)&dev_.w2t, kMlpHidden * kMlpHidden * sizeof(uint16_t));
     dmalloc((void**)&dev_.w3t, kMlpOutputs * kMlpHidden * sizeof(uint16_t));
     dmalloc((void**)&dev_.w2, kMlpHidden * kMlpHidden * sizeof(uint16_t));
@@ -56,7 +58,7 @@ Surrogate::~Surrogate()
     // destructors report nothing: errors here leave nothing to recover
     (void)hipSetDevice(device_);
     if (stream_) (void)hipStreamSynchronize(stream_);
-    void* ptrs[] = {dev_.master, dev_.adam_m, dev_.adam_v, dev_.w1t, dev_.w2t, dev_.w3t, dev_.w2, dev_.w3,
+    void* ptrs[] = {dev_.master, dev_.adam_m, dev_.adam_v, dev_.w1, dev_.w2t, dev_.w3t, dev_.w2, dev_.w3,
                     d_norm_, d_partial_, d_loss_part_, d_grad_, d_loss_, d_idx_};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
