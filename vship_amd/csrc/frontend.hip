@@ -469,10 +469,11 @@ VSHIP_INLINE void chroma_420_left(const FrontColor& fc, const float* s_chroma, f
     const int m = t & 31;
     // staged row e = p * crows + r of plane p is row e of both the staged
     // planes and s_mid (planes back to back), so item it of the thread reads
-    // and writes at fixed offsets from its first one
+    // and writes at fixed offsets from its first one (a rolled loop: unrolled,
+    // the FP16 instantiation spills a register at 6 waves per SIMD)
     const float* src0 = s_chroma + (t >> 5) * ccols + m;
     float* dst0 = s_mid + (t >> 5) * kMidPitch + 2 * m;
-#pragma unroll
+#pragma unroll 1
     for (int it = 0; it < kIters; it++) {
         if ((t >> 5) + kRowStep * it >= 2 * crows) break;
         const float* src = src0 + kRowStep * it * ccols;
@@ -917,23 +918,31 @@ VSHIP_INLINE void store_tile(const FrontJob& job, const PairGeom& geo, int tile_
         }
     }
     // levels 3 and 4: lanes 0..15 and 16..19 of waves 0 and 1 (wave w holds
-    // level-3 rows 2w, 2w + 1 and level-4 row w)
+    // level-3 rows 2w, 2w + 1 and level-4 row w). Both levels' geometry is
+    // read as uniform values and selected per lane: indexing geo by a
+    // lane-dependent level would be per-lane memory loads issued after this
+    // tile's plane stores, and waiting for them (loads and stores share one
+    // in-order counter) would wait for every store to complete.
     if (t < 128) {
         const int lane = t & 63, w = t >> 6;
-        if (lane < 16) {
-            const int x3 = tile_x * 8 + (lane & 7), y3 = tile_y * 4 + 2 * w + (lane >> 3);
-            if (x3 < geo.w[3] && y3 < geo.h[3]) {
-                const uint32_t o = __umul24((uint32_t)y3, (uint32_t)geo.pitch[3]) + (uint32_t)x3;
+        // readfirstlane: the values are uniform; as opaque scalars the
+        // compiler cannot turn their per-lane selection back into addresses
+        auto u = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
+        const int w3 = u(geo.w[3]), h3 = u(geo.h[3]), p3 = u(geo.pitch[3]);
+        const int w4 = u(geo.w[4]), h4 = u(geo.h[4]), p4 = u(geo.pitch[4]);
+        uint32_t o3[3], o4[3];
 #pragma unroll
-                for (int c = 0; c < 3; c++) emit(c, (uint32_t)geo.off[3][c] + o, kFieldL3 + c);
-            }
-        } else if (lane < 20) {
-            const int x4 = tile_x * 4 + (lane - 16), y4 = tile_y * 2 + w;
-            if (x4 < geo.w[4] && y4 < geo.h[4]) {
-                const uint32_t o = __umul24((uint32_t)y4, (uint32_t)geo.pitch[4]) + (uint32_t)x4;
+        for (int c = 0; c < 3; c++) {
+            o3[c] = (uint32_t)u((int)geo.off[3][c]);
+            o4[c] = (uint32_t)u((int)geo.off[4][c]);
+        }
+        const bool l3 = lane < 16;
+        const int x = l3 ? tile_x * 8 + (lane & 7) : tile_x * 4 + (lane - 16);
+        const int y = l3 ? tile_y * 4 + 2 * w + (lane >> 3) : tile_y * 2 + w;
+        if (lane < 20 && x < (l3 ? w3 : w4) && y < (l3 ? h3 : h4)) {
+            const uint32_t o = __umul24((uint32_t)y, (uint32_t)(l3 ? p3 : p4)) + (uint32_t)x;
 #pragma unroll
-                for (int c = 0; c < 3; c++) emit(c, (uint32_t)geo.off[4][c] + o, kFieldL3 + c);
-            }
+            for (int c = 0; c < 3; c++) emit(c, (l3 ? o3[c] : o4[c]) + o, kFieldL3 + c);
         }
     }
 }
@@ -1056,11 +1065,14 @@ VSHIP_INLINE void frontend_tile(const FrontJob& job, const FrontColor* __restric
         unpack_raw(c0, raw, work, P);
         raw = raw_tile<true>(job, 1, c1, tile_x, tile_y);
         compute_tile<SPEC>(job, 0, c0, geo, tile_x, tile_y, work, l1_area, r, P, true);
+        // image 1's samples are unpacked before image 0's stores are issued:
+        // loads and stores share one in-order wait counter, so waiting for the
+        // samples after the stores would wait for the stores to complete
+        unpack_raw(c1, raw, work, P);
         void* o0 = job.out[0];
         store_tile(
             job, geo, tile_x, tile_y, [&](int, uint32_t off, int f) { gstore1<StoreT>(o0, off, field(r, f)); },
             [&](int, uint32_t off, int f) { gstore2<StoreT>(o0, off, field(r, f), field(r, f + 1)); });
-        unpack_raw(c1, raw, work, P);
         compute_tile<SPEC>(job, 1, c1, geo, tile_x, tile_y, work, l1_area + kL1Floats, r, P, true);
         void* o1 = job.out[1];
         store_tile(
@@ -1157,9 +1169,16 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
             return raw_tile<kO32>(j, PHASE, color(j.color[PHASE]), local % j.tiles_x, local / j.tiles_x);
         };
         RawTile raw = {};
+        float P[3][kBH][kBW];
         if (item < end) {
             const FrontJob& job = jobs[front_job_of(jobs, njobs, item)];
             raw = first_raw(job, item - job.tile_start);
+            // phase 0 unpacks each tile's first image before the loop reaches
+            // it: this one here, the next ones before the current tile's
+            // stores (loads and stores share one in-order wait counter, so
+            // waiting for prefetched samples after the stores would wait for
+            // the stores to complete)
+            if constexpr (PHASE == 0) unpack_raw(color(job.color[0]), raw, work, P);
         }
         for (; item < end; item += step) {
             const FrontJob& job = jobs[front_job_of(jobs, njobs, item)];
@@ -1168,7 +1187,6 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
             const int tile_x = local % job.tiles_x, tile_y = local / job.tiles_x;
             const FrontColor c1 = color(job.color[1]);
             TileXyb a, b;
-            float P[3][kBH][kBW];
             const int next = item + step;
             if constexpr (PHASE == 1) {
                 // image 0's FP32 values, stored by the job that computed it,
@@ -1193,18 +1211,21 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
                     });
                 __syncthreads();  // LDS is reused by the next item
             } else {
-                const FrontColor c0 = color(job.color[0]);
-                unpack_raw(c0, raw, work, P);
+                const FrontColor c0 = color(job.color[0]);  // P and the staged chroma hold image 0's samples
                 raw = raw_tile<kO32>(job, 1, c1, tile_x, tile_y);  // in flight during image 0
                 compute_tile<SPEC>(job, 0, c0, geo, tile_x, tile_y, work, l1_area, a, P, true);
 #pragma unroll
                 for (int f = 0; f < kFields; f++) stash[f * kFrontThreads + t] = field(a, f);
                 unpack_raw(c1, raw, work, P);
+                int nxt_color = 0;
                 if (next < end) {  // the next tile's first image, in flight during image 1
                     const FrontJob& nj = jobs[front_job_of(jobs, njobs, next)];
                     raw = first_raw(nj, next - nj.tile_start);
+                    nxt_color = nj.color[0];
                 }
                 compute_tile<SPEC>(job, 1, c1, geo, tile_x, tile_y, work, l1_area + kL1Floats, b, P, true);
+                // compute_tile has passed its level-2 barrier: the chroma staging is free
+                if (next < end) unpack_raw(color(nxt_color), raw, work, P);
                 void* o0 = job.out[0];
                 void* o1 = job.out[1];
                 float* e0 = job.exact0;  // image 0 shared with later jobs: its FP32 values too
