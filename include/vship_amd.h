@@ -82,11 +82,15 @@ typedef enum {
  * 1: 64x16, 2: 32x64, 3: 32x32, 4: 64x64; matrix-core kernels 5: 64x48
  * tiles, 6: 64x64, 7: 64x128, 8: 64x256 streamed strips) and the grids of
  * the blur and front-end kernels in workgroups per CU (1..64; workgroups loop
- * over the items beyond the grid). */
+ * over the items beyond the grid). generic_front: 0 (default) runs each pair
+ * whose two images have 8- or 16-bit YUV 4:2:0 BT.709 layout through the
+ * front-end compiled for that layout, 1 runs every pair through the generic
+ * front-end (the same arithmetic; for comparisons). */
 typedef struct VshipAMD_BatchTuning {
     int blur_config;
     int blur_waves;
     int front_waves;
+    int generic_front;
 } VshipAMD_BatchTuning;
 
 /* A batch context for up to max_pairs pairs per call on gpu_id. XYB planes

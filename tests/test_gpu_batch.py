@@ -147,9 +147,10 @@ def test_layout_instantiation_batch_vs_oracle(kind):
     """A batch whose images all share the 8- or 16-bit YUV 4:2:0 BT.709 layout
     runs the front-end instantiation compiled for that layout: FP32 storage
     equals the single-pair handler (to the final power's last bits) and is
-    within 0.005 of the FP64 oracle; FP16 storage equals the generic front-end bit for bit (forced by
-    adding a pair of another layout to the batch) and is within 0.01 of the oracle on
-    images of 64x64 and more; identity stays 100."""
+    within 0.005 of the FP64 oracle; FP16 storage equals the generic front-end
+    bit for bit (forced with the generic_front tuning, in a batch that also
+    holds a pair of another layout) and is within 0.01 of the oracle on images
+    of 64x64 and more; identity stays 100."""
     sizes = [(131, 75), (64, 64), (200, 117), (17, 9), (1, 1), (320, 180), (40, 30)]
     n = len(sizes) - 1
     dp = DevicePairs(sizes, [kind] * n + [17], seed=40 + kind)  # last pair: 8-bit RGB, never specialised
@@ -159,7 +160,11 @@ def test_layout_instantiation_batch_vs_oracle(kind):
             got32 = b.compute(*dp.args(range(n)))
         with S.SSIMU2Batch(8, S.STORAGE_FP16) as b:
             got16 = b.compute(*dp.args(range(n)))
+            t = b.tuning()
+            b.set_tuning(t["blur_config"], t["blur_waves"], t["front_waves"], generic_front=1)
             generic16 = b.compute(*dp.args())
+            assert b.tuning()["generic_front"] == 1
+            b.set_tuning(t["blur_config"], t["blur_waves"], t["front_waves"])
             cs, _, p1, _, st, _ = dp.args(range(n))
             ident = b.compute(cs, cs, p1, p1, st, st)
         for i, cs_i in enumerate(dp.cs[:n]):
@@ -244,7 +249,7 @@ def test_batch_errors(pairs):
         with pytest.raises(S.VshipError) as e:
             b.compute(*args)
         assert e.value.code == 4
-        for bad in [(BLUR_CONFIGS, 4, 4), (8, 0, 4), (8, 65, 4), (8, 4, 0), (8, 4, 65)]:
+        for bad in [(BLUR_CONFIGS, 4, 4), (8, 0, 4), (8, 65, 4), (8, 4, 0), (8, 4, 65), (8, 4, 4, 2), (8, 4, 4, -1)]:
             with pytest.raises(S.VshipError):
                 b.set_tuning(*bad)
         b.set_tuning(8, 64, 64)  # the largest grids, on a one-pair batch
@@ -388,6 +393,38 @@ def test_shared_reference_pyramid(kinds, storage):
                                         src.strides[s])
                 _say(f"shared ref fp16 vs oracle {src.cs[s].width}x{src.cs[s].height}: {shared[i]:.6f} {want:.6f}")
                 assert abs(shared[i] - want) <= 0.01
+    finally:
+        src.free()
+
+
+@pytest.mark.parametrize("storage", [S.STORAGE_FP32, S.STORAGE_FP16])
+def test_mixed_layouts_run_their_instantiations(storage):
+    """One batch of 8-bit and 10-bit YUV 4:2:0 BT.709 and 8-bit RGB sources,
+    each with pairs that share its reference: the front-end runs one launch per
+    (phase, layout) group, six in all. Scores and features equal those of the
+    generic front-end for every pair (generic_front tuning) bit for bit, and
+    those of each source's pairs computed in a batch of their own."""
+    kinds = (0, 1, 17)
+    src = SharedSources([(160, 90), (97, 61), (64, 64)], kinds, seed=90)
+    try:
+        order = src.order()
+        n = len(order)
+        args = src.args(shared=True)
+        with S.SSIMU2Batch(n, storage) as b:
+            mixed = b.compute(*args)
+            f_mixed = b.features(n)
+            t = b.tuning()
+            b.set_tuning(t["blur_config"], t["blur_waves"], t["front_waves"], generic_front=1)
+            generic = b.compute(*args)
+            f_generic = b.features(n)
+            b.set_tuning(t["blur_config"], t["blur_waves"], t["front_waves"])
+            for s in range(len(src.cs)):
+                idx = [i for i, (s2, _) in enumerate(order) if s2 == s]
+                alone = b.compute(*[[a[i] for i in idx] for a in args])
+                _say(f"mixed layouts source {s} kind {kinds[s]}: {mixed[idx]} alone {alone}")
+                np.testing.assert_array_equal(alone, mixed[idx])
+        np.testing.assert_array_equal(mixed, generic)
+        np.testing.assert_array_equal(f_mixed, f_generic)
     finally:
         src.free()
 

@@ -551,6 +551,7 @@ Vship_Exception VshipAMD_SSIMU2BatchAutotune(VshipAMD_SSIMU2Batch batch, int npa
             chosen->blur_config = t.blur_config;
             chosen->blur_waves = t.blur_waves;
             chosen->front_waves = t.front_waves;
+            chosen->generic_front = t.generic_front;
         }
     });
 }
@@ -562,6 +563,7 @@ Vship_Exception VshipAMD_SSIMU2BatchSetTuning(VshipAMD_SSIMU2Batch batch, VshipA
         t.blur_config = tuning.blur_config;
         t.blur_waves = tuning.blur_waves;
         t.front_waves = tuning.front_waves;
+        t.generic_front = tuning.generic_front;
         with_object(g_batches, batch.id, "VshipAMD_SSIMU2BatchSetTuning", [&](Ssimu2Batch& b) { b.set_tuning(t); });
     });
 }
@@ -575,6 +577,7 @@ Vship_Exception VshipAMD_SSIMU2BatchGetTuning(VshipAMD_SSIMU2Batch batch, VshipA
         tuning->blur_config = t.blur_config;
         tuning->blur_waves = t.blur_waves;
         tuning->front_waves = t.front_waves;
+        tuning->generic_front = t.generic_front;
     });
 }
 

@@ -257,34 +257,37 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
         }
         btiles += L.blur_tiles;
     }
-    // front-end jobs: the pairs that compute both images first (phase 0),
-    // then those whose image 0 a phase-0 job stores (phase 1)
-    int ftiles = 0;
+    // front-end jobs in groups g = phase * 3 + layout instantiation: the pairs
+    // that compute both images (phase 0) before those whose image 0 a phase-0
+    // job stores (phase 1), each phase by layout, so that every pair of a
+    // mixed batch runs its layout's instantiation (one launch per group)
+    std::vector<FrontJob> groups[kFrontGroups];
     for (auto* list : {&own_jobs, &shared_jobs}) {
-        if (list == &shared_jobs) front_shared_ = ftiles;
-        for (FrontJob j : *list) {
+        for (const FrontJob& j : *list) {
+            const FrontColor& c0 = colors_[j.color[0]];
+            const FrontColor& c1 = colors_[j.color[1]];
+            int spec = front_spec(c0, c1);
+            if (spec != kFrontGeneric && (tuning_.generic_front || !front_offsets_fit(j, c0, c1)))
+                spec = kFrontGeneric;
+            groups[(list == &shared_jobs ? 3 : 0) + spec].push_back(j);
+        }
+    }
+    int ftiles = 0;
+    unsigned mask = 0;
+    for (int g = 0; g < kFrontGroups; g++) {
+        front_groups_[g] = ftiles;
+        if (!groups[g].empty()) mask |= 1u << g;
+        for (FrontJob j : groups[g]) {
             const int n = j.tile_start;
             j.tile_start = ftiles;
             ftiles += n;
             fjobs_.push_back(j);
         }
     }
-    // the phase-1 launch is captured only when a job uses it
-    if (shared_jobs.empty() == front_phase1_) {
-        front_phase1_ = !shared_jobs.empty();
-        graph_valid_ = false;
-    }
-    // one front-end instantiation for the whole table: the layout every pair
-    // shares, else the generic one (a change re-captures the graph)
-    int spec = -1;
-    for (const FrontJob& j : fjobs_) {
-        int sj = front_spec(colors_[j.color[0]], colors_[j.color[1]]);
-        if (sj != kFrontGeneric && !front_offsets_fit(j, colors_[j.color[0]], colors_[j.color[1]])) sj = kFrontGeneric;
-        spec = (spec == -1 || spec == sj) ? sj : kFrontGeneric;
-    }
-    if (spec < 0) spec = kFrontGeneric;
-    if (spec != front_spec_) {
-        front_spec_ = spec;
+    front_groups_[kFrontGroups] = ftiles;
+    // the graph holds one launch per non-empty group
+    if (mask != front_mask_) {
+        front_mask_ = mask;
         graph_valid_ = false;
     }
     xyb_elems_ = elems;
@@ -373,7 +376,7 @@ void Ssimu2Batch::ensure_capacity()
     counts[kCountBlurTiles] = btotal;
     counts[kCountBlurJobs] = (int)bmaps.size();
     counts[kCountPairs] = table_pairs_;
-    counts[kCountFrontShared] = front_shared_;
+    for (int g = 0; g <= kFrontGroups; g++) counts[kCountFrontGroup + g] = front_groups_[g];
     auto put = [&](const void* dev, const void* src, size_t bytes) {
         std::memcpy(h_tables_ + (static_cast<const uint8_t*>(dev) - tables_), src, bytes);
     };
@@ -388,22 +391,29 @@ void Ssimu2Batch::ensure_capacity()
     VSHIP_HIP_CHECK(hipMemcpyAsync(tables_, h_tables_, used, hipMemcpyHostToDevice, stream_));
 }
 
-void Ssimu2Batch::launch_kernels(hipStream_t s)
+void Ssimu2Batch::launch_front_groups(hipStream_t s)
 {
     const int fgrid = std::max(1, tuning_.front_waves) * cus_;
-    const int bgrid = (std::max(1, tuning_.blur_waves) * cus_ + 7) / 8 * 8;
     const size_t lds = frontend_max_lds_bytes();
-    if (fp16_) {
-        for (int phase = 0; phase < (front_phase1_ ? 2 : 1); phase++)
+    for (int g = 0; g < kFrontGroups; g++) {
+        if (!(front_mask_ & (1u << g))) continue;
+        if (fp16_)
             VSHIP_HIP_CHECK(
-                launch_frontend<__half>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, front_spec_, phase, s));
-        VSHIP_HIP_CHECK(launch_blur<__half>(tuning_.blur_config, d_bmaps_, d_counts_, bgrid, taps_, partials_, s));
-    } else {
-        for (int phase = 0; phase < (front_phase1_ ? 2 : 1); phase++)
+                launch_frontend<__half>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, g % 3, g / 3, s));
+        else
             VSHIP_HIP_CHECK(
-                launch_frontend<float>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, front_spec_, phase, s));
-        VSHIP_HIP_CHECK(launch_blur<float>(tuning_.blur_config, d_bmaps_, d_counts_, bgrid, taps_, partials_, s));
+                launch_frontend<float>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds, g % 3, g / 3, s));
     }
+}
+
+void Ssimu2Batch::launch_kernels(hipStream_t s)
+{
+    const int bgrid = (std::max(1, tuning_.blur_waves) * cus_ + 7) / 8 * 8;
+    launch_front_groups(s);
+    if (fp16_)
+        VSHIP_HIP_CHECK(launch_blur<__half>(tuning_.blur_config, d_bmaps_, d_counts_, bgrid, taps_, partials_, s));
+    else
+        VSHIP_HIP_CHECK(launch_blur<float>(tuning_.blur_config, d_bmaps_, d_counts_, bgrid, taps_, partials_, s));
     VSHIP_HIP_CHECK(launch_score(d_sjobs_, d_counts_, kScoreGrid, d_maps_, partials_, d_features_, d_scores_, s));
 }
 
@@ -547,7 +557,7 @@ void Ssimu2Batch::features(double* out)
 void Ssimu2Batch::set_tuning(const BatchTuning& t)
 {
     if (t.blur_config < 0 || t.blur_config >= blur_tile_configs() || t.blur_waves < 1 || t.blur_waves > 64 ||
-        t.front_waves < 1 || t.front_waves > 64)
+        t.front_waves < 1 || t.front_waves > 64 || (t.generic_front != 0 && t.generic_front != 1))
         VSHIP_THROW(Vship_BadPointer, "tuning parameters out of range");
     tuning_ = t;
     graph_valid_ = false;
@@ -608,20 +618,11 @@ void Ssimu2Batch::profile(int iters, double* ms_front, double* ms_blur, double* 
     iters = std::max(1, iters);
     hipEvent_t ev[4];
     for (auto& e : ev) VSHIP_HIP_CHECK(hipEventCreate(&e));
-    const int fgrid = std::max(1, tuning_.front_waves) * cus_;
     const int bgrid = (std::max(1, tuning_.blur_waves) * cus_ + 7) / 8 * 8;
-    const size_t lds = frontend_max_lds_bytes();
     double acc[3] = {0, 0, 0};
     for (int it = 0; it < iters; it++) {
         VSHIP_HIP_CHECK(hipEventRecord(ev[0], stream_));
-        for (int phase = 0; phase < (front_phase1_ ? 2 : 1); phase++) {
-            if (fp16_)
-                VSHIP_HIP_CHECK(launch_frontend<__half>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds,
-                                                        front_spec_, phase, stream_));
-            else
-                VSHIP_HIP_CHECK(launch_frontend<float>(d_fjobs_, d_counts_, fgrid, d_colors_, d_geoms_, lds,
-                                                       front_spec_, phase, stream_));
-        }
+        launch_front_groups(stream_);
         VSHIP_HIP_CHECK(hipEventRecord(ev[1], stream_));
         if (fp16_) VSHIP_HIP_CHECK(launch_blur<__half>(tuning_.blur_config, d_bmaps_, d_counts_, bgrid, taps_, partials_, stream_));
         else VSHIP_HIP_CHECK(launch_blur<float>(tuning_.blur_config, d_bmaps_, d_counts_, bgrid, taps_, partials_, stream_));

@@ -33,6 +33,7 @@ struct BatchTuning {
     int blur_config = kDefaultBatchBlurConfig;  // blur tile shape (blur_tile_shape)
     int blur_waves = kDefaultBatchBlurWaves;    // blur workgroups per CU in the grid (1..64)
     int front_waves = kDefaultBatchFrontWaves;  // front-end workgroups per CU in the grid (1..64)
+    int generic_front = 0;                      // 1: no layout instantiation of the front-end
 };
 
 struct BatchInput {
@@ -78,6 +79,7 @@ private:
     void build_tables(const BatchInput& in);
     void ensure_capacity();
     void capture();
+    void launch_front_groups(hipStream_t s);  // one front-end launch per non-empty job group
     void launch_kernels(hipStream_t s);
 
     int device_;
@@ -161,9 +163,8 @@ private:
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
     bool graph_valid_ = false;
-    int front_spec_ = 0;  // FrontSpec of the front-end instantiation in use
-    int front_shared_ = 0;  // first front-end tile of the jobs with a shared image 0
-    bool front_phase1_ = false;  // the batch has such jobs: the front-end runs twice
+    int front_groups_[kFrontGroups + 1] = {};  // first front-end tile of each job group, then the total
+    unsigned front_mask_ = 0;                  // bit g: group g has jobs (a captured launch)
 };
 
 }  // namespace vship

@@ -1127,19 +1127,19 @@ VSHIP_INLINE int front_job_of(const FrontJob* __restrict__ jobs, int njobs, int 
 // The layout instantiations need fewer registers: 80 give 6 waves per SIMD,
 // i.e. three 8-wave workgroups per CU (LDS allows three in both layouts).
 //
-// PHASE 0 covers tiles [0, first_shared), the jobs that compute both images;
-// PHASE 1 the rest, the jobs whose image 0 a phase-0 job stored (launched
-// after phase 0 on the same stream). Separate instantiations keep the
-// image-1-only code out of the register allocation of the common one.
+// Each launch covers the tiles of one job group (CountSlot kCountFrontGroup):
+// the jobs of layout SPEC that compute both images (PHASE 0) or whose image 0
+// a phase-0 job stored (PHASE 1, launched after every phase-0 group on the
+// same stream). Separate instantiations keep the image-1-only code out of the
+// register allocation of the common one.
 template <typename StoreT, int SPEC, int PHASE>
 __global__ __launch_bounds__(kFrontThreads, SPEC == kFrontGeneric ? 4 : 6)
 void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ counts,
                      const FrontColor* __restrict__ colors, const PairGeom* __restrict__ geoms)
 {
     extern __shared__ float lds[];
-    const int first_shared = counts[kCountFrontShared];
-    const int lo = PHASE == 0 ? 0 : first_shared;
-    const int hi = PHASE == 0 ? first_shared : counts[kCountFrontTiles];
+    const int lo = counts[kCountFrontGroup + PHASE * 3 + SPEC];
+    const int hi = counts[kCountFrontGroup + PHASE * 3 + SPEC + 1];
     const int njobs = counts[kCountFrontJobs];
     const int per = (hi - lo + kXcds - 1) / kXcds;
     const int xcd = blockIdx.x % kXcds;

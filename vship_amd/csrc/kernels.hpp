@@ -27,11 +27,11 @@ bool front_offsets_fit(const FrontJob& job, const FrontColor& c0, const FrontCol
 
 // counts: device array indexed by CountSlot. With fewer workgroups than items
 // the kernels loop over items (persistent mode).
-// spec: the FrontSpec every image of the job table shares (kFrontGeneric
-// when they differ).
-// phase 0 runs tiles [0, counts[kCountFrontShared]), phase 1 the rest: the
-// jobs whose image 0 (FrontJob::shared) a phase-0 job stores, which must have
-// completed (stream order).
+// spec, phase: the job group this launch runs, tiles
+// [counts[kCountFrontGroup + g], counts[kCountFrontGroup + g + 1]) with
+// g = phase * 3 + spec: jobs whose images both have layout spec (generic: any
+// layout), and, in phase 1, whose image 0 (FrontJob::shared) a phase-0 job
+// stores, which must have completed (stream order).
 template <typename StoreT>
 hipError_t launch_frontend(const FrontJob* jobs, const int* counts, int grid, const FrontColor* colors,
                            const PairGeom* geoms, size_t lds_bytes, int spec, int phase, hipStream_t stream);

@@ -56,9 +56,13 @@ enum CountSlot : int32_t {
     kCountBlurTiles = 2,
     kCountBlurJobs = 3,
     kCountPairs = 4,
-    kCountFrontShared = 5,  // first front-end tile of the jobs with a shared image 0
-    kCountSlots = 8,
+    // front-end groups g = phase * 3 + FrontSpec (phase 1: the jobs whose
+    // image 0 a phase-0 job stores): group g covers front-end tiles
+    // [counts[kCountFrontGroup + g], counts[kCountFrontGroup + g + 1])
+    kCountFrontGroup = 8,
+    kCountSlots = 16,
 };
+constexpr int kFrontGroups = 6;
 
 enum MapFlags : int32_t {
     kMapSsim = 1,

@@ -101,11 +101,14 @@ class Handler(ctypes.Structure):
 
 
 class BatchTuning(ctypes.Structure):
-    """VshipAMD_BatchTuning: blur tile shape id, blur and front-end WGs per CU."""
-    _fields_ = [("blur_config", ctypes.c_int), ("blur_waves", ctypes.c_int), ("front_waves", ctypes.c_int)]
+    """VshipAMD_BatchTuning: blur tile shape id, blur and front-end WGs per CU,
+    generic_front (1: no layout instantiation of the front-end)."""
+    _fields_ = [("blur_config", ctypes.c_int), ("blur_waves", ctypes.c_int), ("front_waves", ctypes.c_int),
+                ("generic_front", ctypes.c_int)]
 
     def as_dict(self):
-        return {"blur_config": self.blur_config, "blur_waves": self.blur_waves, "front_waves": self.front_waves}
+        return {"blur_config": self.blur_config, "blur_waves": self.blur_waves, "front_waves": self.front_waves,
+                "generic_front": self.generic_front}
 
 
 class Version(ctypes.Structure):
@@ -444,8 +447,9 @@ class SSIMU2Batch:
                                                       ctypes.byref(t), ctypes.byref(ms)))
         return t.as_dict(), ms.value
 
-    def set_tuning(self, blur_config: int, blur_waves: int, front_waves: int) -> None:
-        check(self._lib.VshipAMD_SSIMU2BatchSetTuning(self.handle, BatchTuning(blur_config, blur_waves, front_waves)))
+    def set_tuning(self, blur_config: int, blur_waves: int, front_waves: int, generic_front: int = 0) -> None:
+        check(self._lib.VshipAMD_SSIMU2BatchSetTuning(
+            self.handle, BatchTuning(blur_config, blur_waves, front_waves, generic_front)))
 
     def tuning(self) -> dict:
         t = BatchTuning()
