@@ -277,6 +277,11 @@ def test_error_paths_on_gpu():
     with pytest.raises(S.VshipError) as e:
         S.SSIMU2Handler(S.colorspace(64, 64), S.colorspace(64, 65))
     assert e.value.code == 4  # DifferingInputType
+    # stored planes beyond 2^30 elements (the kernels' 32-bit byte offsets):
+    # rejected before any allocation
+    with pytest.raises(S.VshipError) as e:
+        S.SSIMU2Handler(S.colorspace(32768, 32768), S.colorspace(32768, 32768))
+    assert e.value.code == 1 and "2^30" in str(e.value)  # OutOfVRAM
     cs = S.colorspace(32, 32)
     hd = S.SSIMU2Handler(cs, cs)
     hd.close()

@@ -216,7 +216,7 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
                 lj.color[0] = j.color[k];
                 lj.width = (int32_t)sp.width;
                 lj.height = (int32_t)sp.height;
-                lj.tiles_x = (int32_t)((sp.width + kFrontTileW - 1) / kFrontTileW);
+                set_front_tiles(lj, (int)((sp.width + kFrontTileW - 1) / kFrontTileW));
                 resamples_.push_back(r);
                 rjobs_.push_back(lj);
                 for (int p = 0; p < 3; p++) {
@@ -230,7 +230,7 @@ void Ssimu2Batch::build_tables(const BatchInput& in)
         }
         j.width = (int32_t)s1.final_w;
         j.height = (int32_t)s1.final_h;
-        j.tiles_x = L.front_tiles_x;
+        set_front_tiles(j, L.front_tiles_x);
         j.tile_start = L.front_tiles_per_image;  // tile count until the jobs are ordered
         j.pair = i;
         (j.shared ? shared_jobs : own_jobs).push_back(j);

@@ -346,6 +346,9 @@ PairLayout make_pair_layout(int w, int h, int blur_config)
         ch = half_up(ch);
     }
     L.geom.image_elems = off;
+    // the kernels address an image's stored planes with 32-bit byte offsets
+    if (off >= ((int64_t)1 << 30))
+        VSHIP_THROW(Vship_OutOfVRAM, "image too large: its stored XYB planes exceed 2^30 elements");
     for (int sc = 0; sc < kUsedScales; sc++)
         for (int p = 0; p < 3; p++)
             if ((L.geom.off[sc][p] >= 0) != kStored[sc][p])
@@ -441,7 +444,7 @@ Ssimu2Handler::Ssimu2Handler(const Vship_Colorspace_t& src, const Vship_Colorspa
     }
     j.width = width_;
     j.height = height_;
-    j.tiles_x = layout_.front_tiles_x;
+    set_front_tiles(j, layout_.front_tiles_x);
     j.tile_start = 0;
     j.pair = 0;
 
@@ -471,7 +474,7 @@ Ssimu2Handler::Ssimu2Handler(const Vship_Colorspace_t& src, const Vship_Colorspa
         lj.color[0] = i;
         lj.width = (int32_t)W;
         lj.height = (int32_t)H;
-        lj.tiles_x = (int32_t)((W + kFrontTileW - 1) / kFrontTileW);
+        set_front_tiles(lj, (int)((W + kFrontTileW - 1) / kFrontTileW));
         for (int p = 0; p < 3; p++) {
             j.plane[i][p] = reinterpret_cast<const uint8_t*>(cur + p * tw * th + sp.crop_top * tw + sp.crop_left);
             j.stride[i][p] = tw * (int64_t)sizeof(float);

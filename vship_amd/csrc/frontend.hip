@@ -201,6 +201,7 @@ VSHIP_INLINE void stage_chroma(const uint8_t* const* plane, const int64_t* pstri
         const bool aligned = plane_aligned<KIND>(base, stride);
 #pragma unroll
         for (int it = 0; it < kChromaIters; it++) {
+            if (it * kFrontThreads >= plane_elems) break;  // uniform (constant in the layout instantiations)
             const int e = min(t + it * kFrontThreads, plane_elems - 1);
             const int r = (int)(((float)e + 0.5f) * inv_cols);
             const int c = e - r * ccols;
@@ -270,7 +271,9 @@ VSHIP_INLINE void raw_tile_kind(const FrontJob& job, int k, const FrontColor& fc
             const bool aligned = plane_aligned<KIND>(plane[1 + p], pstride[1 + p]);
 #pragma unroll
             for (int it = 0; it < kChromaIters; it++) {
-                // unconditional (items past the plane reload its last sample)
+                // every item of the thread's rounds with items in the plane
+                // loads (items past the plane reload its last sample)
+                if (it * kFrontThreads >= plane_elems) break;
                 const int e = min(t + it * kFrontThreads, plane_elems - 1);
                 const int rr = (int)(((float)e + 0.5f) * inv_cols);
                 const int c = e - rr * ccols;
@@ -947,21 +950,32 @@ VSHIP_INLINE void store_tile(const FrontJob& job, const PairGeom& geo, int tile_
     }
 }
 
-// Global-memory stores of one or two consecutive plane elements.
+// Global-memory stores and loads of one or two consecutive plane elements at
+// element offset `off` of an image's plane buffer. The byte offset is formed
+// in 32 bits (the host keeps an image's stored elements below 2^30), so the
+// access takes the buffer base as its scalar address and the offset as a
+// 32-bit lane register, with no per-lane 64-bit address arithmetic.
 typedef float f32x2_t __attribute__((ext_vector_type(2)));
 typedef _Float16 f16x2_t __attribute__((ext_vector_type(2)));
+
+template <typename T>
+VSHIP_INLINE __attribute__((address_space(1))) T* elem_ptr(const void* base, uint32_t off)
+{
+    return (__attribute__((address_space(1))) T*)((__attribute__((address_space(1))) uint8_t*)base +
+                                                  off * (uint32_t)sizeof(T));
+}
 
 template <typename StoreT>
 VSHIP_INLINE void gstore1(void* base, uint32_t off, float v);
 template <>
 VSHIP_INLINE void gstore1<float>(void* base, uint32_t off, float v)
 {
-    ((__attribute__((address_space(1))) float*)base)[off] = v;
+    *elem_ptr<float>(base, off) = v;
 }
 template <>
 VSHIP_INLINE void gstore1<__half>(void* base, uint32_t off, float v)
 {
-    ((__attribute__((address_space(1))) _Float16*)base)[off] = (_Float16)v;
+    *elem_ptr<_Float16>(base, off) = (_Float16)v;
 }
 
 template <typename StoreT>
@@ -969,26 +983,24 @@ VSHIP_INLINE void gstore2(void* base, uint32_t off, float v0, float v1);
 template <>
 VSHIP_INLINE void gstore2<float>(void* base, uint32_t off, float v0, float v1)
 {
-    *(__attribute__((address_space(1))) f32x2_t*)((__attribute__((address_space(1))) float*)base + off) =
-        f32x2_t{v0, v1};
+    *(__attribute__((address_space(1))) f32x2_t*)elem_ptr<float>(base, off) = f32x2_t{v0, v1};
 }
 template <>
 VSHIP_INLINE void gstore2<__half>(void* base, uint32_t off, float v0, float v1)
 {
-    *(__attribute__((address_space(1))) f16x2_t*)((__attribute__((address_space(1))) _Float16*)base + off) =
-        f16x2_t{(_Float16)v0, (_Float16)v1};
+    *(__attribute__((address_space(1))) f16x2_t*)elem_ptr<_Float16>(base, off) = f16x2_t{(_Float16)v0, (_Float16)v1};
 }
 
-// Global-memory loads of one or two consecutive FP32 elements of a plane
-// layout (the FP32 values of an image computed by an earlier job).
+// FP32 elements of a plane layout (the FP32 values of an image computed by an
+// earlier job).
 VSHIP_INLINE float gload1(const float* base, uint32_t off)
 {
-    return ((const __attribute__((address_space(1))) float*)base)[off];
+    return *elem_ptr<const float>(base, off);
 }
 
 VSHIP_INLINE void gload2(const float* base, uint32_t off, float& v0, float& v1)
 {
-    const f32x2_t v = *(const __attribute__((address_space(1))) f32x2_t*)((const __attribute__((address_space(1))) float*)base + off);
+    const f32x2_t v = *(const __attribute__((address_space(1))) f32x2_t*)elem_ptr<const float>(base, off);
     v0 = v[0];
     v1 = v[1];
 }
@@ -1150,7 +1162,9 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
         for (; item < end; item += step) {
             const FrontJob& job = jobs[front_job_of(jobs, njobs, item)];
             const int local = item - job.tile_start;
-            frontend_tile<StoreT, SPEC>(job, colors, geoms[job.pair], local % job.tiles_x, local / job.tiles_x, lds);
+            int tile_x, tile_y;
+            front_tile_xy(job, local, tile_x, tile_y);
+            frontend_tile<StoreT, SPEC>(job, colors, geoms[job.pair], tile_x, tile_y, lds);
             __syncthreads();  // LDS is reused by the next item
         }
     } else {
@@ -1166,7 +1180,9 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
         };
         // raw samples of a tile's first image: image 1 when image 0 is shared
         auto first_raw = [&](const FrontJob& j, int local) {
-            return raw_tile<kO32>(j, PHASE, color(j.color[PHASE]), local % j.tiles_x, local / j.tiles_x);
+            int tile_x, tile_y;
+            front_tile_xy(j, local, tile_x, tile_y);
+            return raw_tile<kO32>(j, PHASE, color(j.color[PHASE]), tile_x, tile_y);
         };
         RawTile raw = {};
         float P[3][kBH][kBW];
@@ -1184,7 +1200,8 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
             const FrontJob& job = jobs[front_job_of(jobs, njobs, item)];
             const PairGeom& geo = geoms[job.pair];
             const int local = item - job.tile_start;
-            const int tile_x = local % job.tiles_x, tile_y = local / job.tiles_x;
+            int tile_x, tile_y;
+            front_tile_xy(job, local, tile_x, tile_y);
             const FrontColor c1 = color(job.color[1]);
             TileXyb a, b;
             const int next = item + step;
@@ -1261,7 +1278,9 @@ void frontend_single_kernel(FrontJob job, int tiles, const FrontColor* __restric
     const int per = (tiles + kXcds - 1) / kXcds;
     const int item = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
     if (item >= tiles || (int)(blockIdx.x / kXcds) >= per) return;
-    frontend_tile<StoreT, SPEC>(job, colors, geoms[job.pair], item % job.tiles_x, item / job.tiles_x, lds);
+    int tile_x, tile_y;
+    front_tile_xy(job, item, tile_x, tile_y);
+    frontend_tile<StoreT, SPEC>(job, colors, geoms[job.pair], tile_x, tile_y, lds);
 }
 
 // Linear RGB of image 0 of one job, stored as three W x H planes with row

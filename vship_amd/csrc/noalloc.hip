@@ -134,7 +134,7 @@ double compute_ssimu2_noalloc(const Vship_Colorspace_t& src, const Vship_Colorsp
     }
     job.width = (int32_t)s1.width;
     job.height = (int32_t)s1.height;
-    job.tiles_x = L.front_tiles_x;
+    set_front_tiles(job, L.front_tiles_x);
     int spec = front_spec(s1.fc, s2.fc);
     if (spec != kFrontGeneric && !front_offsets_fit(job, s1.fc, s2.fc)) spec = kFrontGeneric;
 

@@ -117,7 +117,9 @@ struct FrontJob {
     void* out[2];                // XYB planes of each image (StoreT elements)
     int32_t color[2];            // index into the FrontColor table
     int32_t width, height;       // level-0 size
-    int32_t tiles_x;             // 64-wide tiles along x
+    int32_t tiles_x;             // 64-wide tiles along x (set with set_front_tiles)
+    uint32_t tiles_x_magic;      // local / tiles_x = (local * magic) >> shift (front_tile_xy)
+    int32_t tiles_x_shift;
     int32_t tile_start;          // first global tile index of this job
     int32_t pair;                // index into the PairGeom table
     int32_t shared;              // 1: image 0 was computed by an earlier job
@@ -129,6 +131,30 @@ struct FrontJob {
     // otherwise
     float* exact0;
 };
+
+// Sets a job's tiles_x and the round-up reciprocal that divides by it:
+// magic = ceil(2^(31 + l) / tiles_x) with 2^(l-1) < tiles_x <= 2^l lies below
+// 2^32 and exceeds 2^(31 + l) / tiles_x by less than 2^l / tiles_x, so
+// (local * magic) >> (31 + l) is exactly local / tiles_x for every local below
+// 2^31.
+inline void set_front_tiles(FrontJob& j, int tiles_x)
+{
+    int l = 0;
+    while ((1u << l) < (uint32_t)tiles_x) l++;
+    j.tiles_x = tiles_x;
+    j.tiles_x_magic = (uint32_t)((((uint64_t)1 << (31 + l)) + (uint64_t)tiles_x - 1) / (uint64_t)tiles_x);
+    j.tiles_x_shift = 31 + l;
+}
+
+// Tile (x, y) of tile `local` of a job, from the job's reciprocal: with the
+// job and local uniform across the workgroup the division is three scalar
+// instructions instead of a float-reciprocal sequence on the vector unit.
+__device__ inline void front_tile_xy(const FrontJob& j, int local, int& tile_x, int& tile_y)
+{
+    const int q = (int)(((uint64_t)(uint32_t)local * j.tiles_x_magic) >> j.tiles_x_shift);
+    tile_y = q;
+    tile_x = local - q * j.tiles_x;
+}
 
 // The stored XYB planes per scale (plane X, Y, B), fixed by the weight table:
 // a (plane, scale) is stored iff one of its three maps has an L1 or L4 weight
