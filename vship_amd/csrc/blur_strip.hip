@@ -99,6 +99,17 @@ __device__ inline const __attribute__((address_space(1))) T* global_ptr(const vo
     return (const __attribute__((address_space(1))) T*)p;
 }
 
+// Element i of a plane as a 32-bit byte offset from the plane pointer (an
+// image's stored planes hold fewer than 2^30 elements, so the offset fits):
+// the load takes the pointer as its scalar base and the offset as one lane
+// register, with no per-lane 64-bit address arithmetic.
+template <typename T>
+__device__ inline T global_at(const void* p, uint32_t i)
+{
+    return *(const __attribute__((address_space(1))) T*)((const __attribute__((address_space(1))) uint8_t*)p +
+                                                         i * (uint32_t)sizeof(T));
+}
+
 // lo = fp16(x - hi) of two values whose packed FP16 hi parts are hh: x - hi
 // is exact in FP32 (hi is x truncated to 11 bits) and rounded once to FP16.
 // one is 1.0 hidden from the optimiser: fma(x, one, -hi) is then selected as
@@ -277,8 +288,6 @@ __device__ inline void stage_load(const void* A, const void* B, const StripGeom&
                                   const typename StageRegs<StoreT>::Vec& fa, const typename StageRegs<StoreT>::Vec& fb)
 {
     typedef typename StageRegs<StoreT>::Vec Vec;
-    const auto* ga = global_ptr<Vec>(A);
-    const auto* gb = global_ptr<Vec>(B);
     const int t = threadIdx.x;
 #pragma unroll
     for (int it = 0; it < kStageIters; it++) {
@@ -290,8 +299,8 @@ __device__ inline void stage_load(const void* A, const void* B, const StripGeom&
         if (ok) {
             // vector index inside the plane (< 2^31 elements per plane)
             const uint32_t o = ((uint32_t)gy * (uint32_t)g.pitch + (uint32_t)gx) >> 2;
-            va = ga[o];
-            vb = gb[o];
+            va = global_at<Vec>(A, o);
+            vb = global_at<Vec>(B, o);
             if (gx + 4 > g.w) fill_tail(va, vb, g.w - gx, fa, fb);
         }
         r.a[it] = va;

@@ -380,49 +380,44 @@ VSHIP_INLINE void apply_transfer(int trc, float P[3][kBH][kBW])
 struct TileXyb {
     float y0[kBH][kBW];              // level 0: Y of the thread's block
     float x1[kBH / 2], y1[kBH / 2];  // level 1: X, Y of the block's parents
-    float x2 = 0.0f, y2 = 0.0f;      // level 2 (threads < 128)
-    float l3[3] = {};                // levels 3 and 4 (lanes 0..19 of waves 0, 1): X, Y, B
+    float x2[2] = {}, y2[2] = {};    // level 2: X, Y of two adjacent pixels (wave 0)
+    float l3[3] = {};                // level 3 or 4 (some lanes of wave 0): X, Y, B
 };
 
-// Flat field index of a TileXyb value (the FP16 path's LDS stash is
-// field-major).
+// Flat field index of a TileXyb value.
 constexpr int kFieldY0 = 0;
 constexpr int kFieldX1 = kFieldY0 + kBH * kBW;
 constexpr int kFieldY1 = kFieldX1 + kBH / 2;
 constexpr int kFieldX2 = kFieldY1 + kBH / 2;
-constexpr int kFieldY2 = kFieldX2 + 1;
-constexpr int kFieldL3 = kFieldY2 + 1;
+constexpr int kFieldY2 = kFieldX2 + 2;
+constexpr int kFieldL3 = kFieldY2 + 2;
 constexpr int kFields = kFieldL3 + 3;
-static_assert(kFields * kFrontThreads == kFrontStashFloats, "front-end stash size");
+// The FP16 path's LDS stash is field-major, one float per thread and field;
+// the second level-2 pixel of thread t < 64 takes thread t + 64's slot of the
+// same row (threads from 64 on hold no level-2 values), so the stash has two
+// rows fewer than there are fields.
+constexpr int kStashRows = kFields - 2;
+static_assert(kStashRows * kFrontThreads == kFrontStashFloats, "front-end stash size");
+static_assert(kFrontThreads >= 128, "level-2 pairs in the stash");
+
+VSHIP_INLINE int stash_slot(int f, int t)
+{
+    if (f < kFieldX2) return f * kFrontThreads + t;
+    if (f < kFieldL3) return (kFieldX2 + ((f - kFieldX2) >> 1)) * kFrontThreads + t + 64 * ((f - kFieldX2) & 1);
+    return (f - 2) * kFrontThreads + t;
+}
+
+// Whether thread t holds field f (the rest are not stored).
+VSHIP_INLINE bool holds_field(int f, int t) { return f < kFieldX2 || t < 64; }
 
 VSHIP_INLINE float field(const TileXyb& r, int f)
 {
     if (f < kFieldX1) return r.y0[(f - kFieldY0) / kBW][(f - kFieldY0) % kBW];
     if (f < kFieldY1) return r.x1[f - kFieldX1];
     if (f < kFieldX2) return r.y1[f - kFieldY1];
-    if (f == kFieldX2) return r.x2;
-    if (f == kFieldY2) return r.y2;
+    if (f < kFieldY2) return r.x2[f - kFieldX2];
+    if (f < kFieldL3) return r.y2[f - kFieldY2];
     return r.l3[f - kFieldL3];
-}
-
-// Level s+1 pixel from the 2x2 parents at (2lx, 2ly) of a level stored in LDS
-// as [3][rows][cols + 1], with the edge clamp min(2x+1, w-1)
-// (downsample.hpp:3-19); (gx2, gy2) are the parents' global coordinates.
-VSHIP_INLINE void down_lds(const float* src, int cols, int rows, int lx, int ly, int gx2, int gy2, int w, int h,
-                           float out[3])
-{
-    const int pitch = cols + 1;
-    const int xb = (gx2 + 1 < w) ? 2 * lx + 1 : 2 * lx;
-    const int yb = (gy2 + 1 < h) ? 2 * ly + 1 : 2 * ly;
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const float* b = src + c * rows * pitch;
-        float s = b[(2 * ly) * pitch + 2 * lx];
-        s += b[yb * pitch + 2 * lx];
-        s += b[(2 * ly) * pitch + xb];
-        s += b[yb * pitch + xb];
-        out[c] = s * 0.25f;
-    }
 }
 
 // Chroma of the thread's 2x2 block when no axis is subsampled more than 2x:
@@ -448,7 +443,7 @@ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 constexpr int kChromaLdsMax =
     cmax(cmax(cmax(chroma_lds_floats(1, 1), chroma_lds_floats(1, 0)), cmax(chroma_lds_floats(2, 0), chroma_lds_floats(2, 1))),
          chroma_lds_floats(0, 1));
-constexpr int kL1Floats = 3 * 16 * 33;
+constexpr int kL1Floats = 3 * 16 * 32;  // level 1 of one image: [3][16 rows][32]
 
 VSHIP_INLINE float dot5(const float w[5], const float v[5])
 {
@@ -748,38 +743,12 @@ VSHIP_INLINE void to_linear(const FrontJob& job, int k, const FrontColor& fc, in
     }
 }
 
-// Cross-lane partners of the level-2..4 reduction.
+// Cross-lane partners of the level-3 and level-4 sums: lane ^ 1 and lane ^ 8
+// (a row rotation by 8 within each 16-lane row) through DPP, lane ^ 16
+// through the LDS crossbar's swizzle (no LDS memory).
 VSHIP_INLINE float lane_xor1(float v) { return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xf, 0xf, false)); }
-VSHIP_INLINE float lane_xor2(float v) { return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xf, 0xf, false)); }
+VSHIP_INLINE float lane_ror8(float v) { return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x128, 0xf, 0xf, false)); }
 VSHIP_INLINE float lane_xor16(float v) { return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F)); }
-VSHIP_INLINE float lane_xor32(float v) { return __shfl_xor(v, 32, 64); }
-
-// Next pyramid level from 2x2 parents held by lane pairs (x partner via fx,
-// y partner via fy) with the edge clamp min(2x+1, w-1) of
-// downsample.hpp:13-16: a parent beyond the level's edge takes its inside
-// partner's value. Every lane of a 2x2 lane group ends with the mean, summed
-// in the reference's order (p00 + p10 + p01 + p11).
-template <typename FX, typename FY>
-VSHIP_INLINE void down_pairs(float v[3], bool x_out, bool y_out, FX fx, FY fy)
-{
-    // the lane group's base lane (even x, even y) ends with the mean of its
-    // 2x2 parents, summed in the reference's order p00 + p10 + p01 + p11;
-    // lanes beyond the edge first take their partner's value (the clamp)
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        float a = v[c];
-        const float ax = fx(a);
-        a = x_out ? ax : a;
-        const float ay = fy(a);
-        a = y_out ? ay : a;
-        const float p10 = fy(a), p01 = fx(a), p11 = fx(p10);
-        float s = a;
-        s += p10;
-        s += p01;
-        s += p11;
-        v[c] = s * 0.25f;
-    }
-}
 
 // P holds the image's unpacked plane values when preloaded (see to_linear).
 template <int SPEC = kFrontGeneric>
@@ -833,45 +802,88 @@ VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc,
 
     // ---- levels 2, 3, 4 (32x16 -> 16x8 -> 8x4 -> 4x2 per tile): level 1
     // through LDS (its own region per image, so no barrier protects it from
-    // the next image), then waves 0 and 1 hold one level-2 pixel per lane and
-    // reduce levels 3 and 4 across lanes with the same edge clamp
-    float* s_l1 = lds_l1;  // [3][16][33]
+    // the next image), then wave 0 alone: lane 8r + c holds the level-2
+    // pixels (2c, r) and (2c + 1, r); level 3 adds the lane's pair and the
+    // next row's pair (lane + 8) in lanes of even r, level 4 the level-3
+    // pixels of lanes ^ 1, ^ 16 and ^ 17 in lanes 0, 2, 4, 6, 32, ..., 38.
+    // Every sum is in the reference's order with its edge clamp
+    // (downsample.hpp:13-17): p(x, y) + p(x, y + 1) + p(x + 1, y) +
+    // p(x + 1, y + 1), a parent beyond the level's edge replaced by the one
+    // inside.
+    float* s_l1 = lds_l1;  // [3][16][32]
 #pragma unroll
     for (int j = 0; j < kBH / 2; j++)
 #pragma unroll
-        for (int c = 0; c < 3; c++) s_l1[(c * 16 + ty * (kBH / 2) + j) * 33 + tx] = L1[j][c];
+        for (int c = 0; c < 3; c++) s_l1[(c * 16 + ty * (kBH / 2) + j) * 32 + tx] = L1[j][c];
     __syncthreads();
-    if (t < 128) {
-        const int lx = t & 15, ly = t >> 4;  // level-2 pixel of the lane
-        float L2[3];
-        down_lds(s_l1, 32, 16, lx, ly, tile_x * 32 + 2 * lx, tile_y * 16 + 2 * ly, geo.w[1], geo.h[1], L2);
-        {
-            const XYB q = to_xyb(L2[0], L2[1], L2[2], false);
-            res.x2 = q.x;
-            res.y2 = q.y;
+    if (t < 64) {
+        const int c = t & 7, r = t >> 3;
+        // level-1 parents: columns 4c .. 4c + 3 of rows 2r, 2r + 1
+        const int gx1 = tile_x * 32 + 4 * c, gy1 = tile_y * 16 + 2 * r;
+        const bool x1in = gx1 + 1 < geo.w[1], x3in = gx1 + 3 < geo.w[1], y1in = gy1 + 1 < geo.h[1];
+        float L2[2][3];
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++) {
+            const float4 a = *reinterpret_cast<const float4*>(s_l1 + (ch * 16 + 2 * r) * 32 + 4 * c);
+            const float4 b0 = *reinterpret_cast<const float4*>(s_l1 + (ch * 16 + 2 * r + 1) * 32 + 4 * c);
+            const float4 b = y1in ? b0 : a;
+            float s0 = a.x;
+            s0 += b.x;
+            s0 += x1in ? a.y : a.x;
+            s0 += x1in ? b.y : b.x;
+            L2[0][ch] = s0 * 0.25f;
+            float s1 = a.z;
+            s1 += b.z;
+            s1 += x3in ? a.w : a.z;
+            s1 += x3in ? b.w : b.z;
+            L2[1][ch] = s1 * 0.25f;
         }
-        // level 3: x partner lane ^ 1, y partner lane ^ 16; level 4: lane ^ 2
-        // and lane ^ 32 (each wave holds four level-2 rows)
-        float L3[3] = {L2[0], L2[1], L2[2]};
-        down_pairs(L3, tile_x * 16 + lx >= geo.w[2], tile_y * 8 + ly >= geo.h[2], lane_xor1, lane_xor16);
-        float L4[3] = {L3[0], L3[1], L3[2]};
-        down_pairs(L4, tile_x * 8 + (lx >> 1) >= geo.w[3], tile_y * 4 + (ly >> 1) >= geo.h[3], lane_xor2,
-                   lane_xor32);
-        // gather the wave's 16 level-3 and 4 level-4 pixels into lanes 0..19
-        // for one XYB pass: level 3 from its base lane (even, even), level 4
-        // moved one lane right of its base lane (x % 4 == 1), so that no lane
-        // serves both
-        const int lane = t & 63;
-        const bool l4src = (lx & 3) == 1 && (ly & 3) == 0;
-        const int src = lane < 16 ? 32 * (lane >> 3) + 2 * (lane & 7) : (lane < 20 ? 4 * (lane - 16) + 1 : lane);
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const XYB q = to_xyb(L2[i][0], L2[i][1], L2[i][2], false);
+            res.x2[i] = q.x;
+            res.y2[i] = q.y;
+        }
+        // level 3 pixel (c, r / 2) in lanes of even r
+        const bool x2in = tile_x * 16 + 2 * c + 1 < geo.w[2], y2in = tile_y * 8 + r + 1 < geo.h[2];
+        float L3[3];
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++) {
+            const float a0 = L2[0][ch], a1 = x2in ? L2[1][ch] : a0;
+            const float n0 = lane_ror8(L2[0][ch]), n1 = lane_ror8(L2[1][ch]);
+            const float b0 = y2in ? n0 : a0, b1 = y2in ? (x2in ? n1 : n0) : a1;
+            float v = a0;
+            v += b0;
+            v += a1;
+            v += b1;
+            L3[ch] = v * 0.25f;
+        }
+        // level 4 pixel (c / 2, r / 4) in lanes of even c and r % 4 == 0
+        const bool x3in4 = tile_x * 8 + c + 1 < geo.w[3], y3in = tile_y * 4 + (r >> 1) + 1 < geo.h[3];
+        float L4[3];
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++) {
+            const float p00 = L3[ch];
+            const float py = lane_xor16(p00);
+            const float px = lane_xor1(p00), pxy = lane_xor1(py);
+            const float p01 = y3in ? py : p00;
+            const float p10 = x3in4 ? px : p00;
+            const float p11 = y3in ? (x3in4 ? pxy : py) : p10;
+            float v = p00;
+            v += p01;
+            v += p10;
+            v += p11;
+            L4[ch] = v * 0.25f;
+        }
+        // one XYB pass: lanes of odd r take the level-3 pixel of lane - 8,
+        // the level-4 lanes keep theirs
         float G[3];
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const float l4m = lane_xor1(L4[c]);
-            G[c] = __shfl(l4src ? l4m : L3[c], src, 64);
+        for (int ch = 0; ch < 3; ch++) {
+            const float l3 = lane_ror8(L3[ch]);
+            G[ch] = (r & 1) ? l3 : L4[ch];
         }
         const XYB q = to_xyb(G[0], G[1], G[2], true);
-        // lanes 0..15 store them as level 3, lanes 16..19 as level 4
         res.l3[0] = q.x; res.l3[1] = q.y; res.l3[2] = q.b;
     }
 }
@@ -879,8 +891,9 @@ VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc,
 // Writes the tile's stored planes. emit(plane, element offset, k) writes
 // plane element `off` (32-bit: an image buffer holds fewer than 2^31
 // elements) from field value k of the thread's TileXyb; emit2(plane, off, k)
-// writes fields k and k + 1 to elements off and off + 1 (level-0 row pairs:
-// plane offsets, row pitches and x0 are even, so the pair is aligned).
+// writes fields k and k + 1 to elements off and off + 1 (level-0 and level-2
+// pixel pairs: plane offsets, row pitches and their first columns are even,
+// so the pair is aligned).
 template <typename Emit, typename Emit2>
 VSHIP_INLINE void store_tile(const FrontJob& job, const PairGeom& geo, int tile_x, int tile_y, Emit emit,
                              Emit2 emit2)
@@ -912,22 +925,27 @@ VSHIP_INLINE void store_tile(const FrontJob& job, const PairGeom& geo, int tile_
             emit(1, (uint32_t)geo.off[1][1] + o, kFieldY1 + j);
         }
     }
-    if (t < 128) {
-        const int x2 = tile_x * 16 + (t & 15), y2 = tile_y * 8 + (t >> 4);
-        if (x2 < geo.w[2] && y2 < geo.h[2]) {
-            const uint32_t o = __umul24((uint32_t)y2, (uint32_t)geo.pitch[2]) + (uint32_t)x2;
-            emit(0, (uint32_t)geo.off[2][0] + o, kFieldX2);
-            emit(1, (uint32_t)geo.off[2][1] + o, kFieldY2);
-        }
-    }
-    // levels 3 and 4: lanes 0..15 and 16..19 of waves 0 and 1 (wave w holds
-    // level-3 rows 2w, 2w + 1 and level-4 row w). Both levels' geometry is
+    // levels 2 to 4: wave 0 (lane 8r + c). Level 2: pixels (2c, r) and
+    // (2c + 1, r), a pair of consecutive elements (even column). Levels 3
+    // and 4: lanes of odd r hold level-3 pixel (c, r / 2), lanes of even c
+    // and r % 4 == 0 level-4 pixel (c / 2, r / 4). Both levels' geometry is
     // read as uniform values and selected per lane: indexing geo by a
     // lane-dependent level would be per-lane memory loads issued after this
     // tile's plane stores, and waiting for them (loads and stores share one
     // in-order counter) would wait for every store to complete.
-    if (t < 128) {
-        const int lane = t & 63, w = t >> 6;
+    if (t < 64) {
+        const int c = t & 7, r = t >> 3;
+        const int x2 = tile_x * 16 + 2 * c, y2 = tile_y * 8 + r;
+        if (y2 < geo.h[2] && x2 < geo.w[2]) {
+            const uint32_t o = __umul24((uint32_t)y2, (uint32_t)geo.pitch[2]) + (uint32_t)x2;
+            if (x2 + 1 < geo.w[2]) {
+                emit2(0, (uint32_t)geo.off[2][0] + o, kFieldX2);
+                emit2(1, (uint32_t)geo.off[2][1] + o, kFieldY2);
+            } else {
+                emit(0, (uint32_t)geo.off[2][0] + o, kFieldX2);
+                emit(1, (uint32_t)geo.off[2][1] + o, kFieldY2);
+            }
+        }
         // readfirstlane: the values are uniform; as opaque scalars the
         // compiler cannot turn their per-lane selection back into addresses
         auto u = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
@@ -935,17 +953,18 @@ VSHIP_INLINE void store_tile(const FrontJob& job, const PairGeom& geo, int tile_
         const int w4 = u(geo.w[4]), h4 = u(geo.h[4]), p4 = u(geo.pitch[4]);
         uint32_t o3[3], o4[3];
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            o3[c] = (uint32_t)u((int)geo.off[3][c]);
-            o4[c] = (uint32_t)u((int)geo.off[4][c]);
+        for (int ch = 0; ch < 3; ch++) {
+            o3[ch] = (uint32_t)u((int)geo.off[3][ch]);
+            o4[ch] = (uint32_t)u((int)geo.off[4][ch]);
         }
-        const bool l3 = lane < 16;
-        const int x = l3 ? tile_x * 8 + (lane & 7) : tile_x * 4 + (lane - 16);
-        const int y = l3 ? tile_y * 4 + 2 * w + (lane >> 3) : tile_y * 2 + w;
-        if (lane < 20 && x < (l3 ? w3 : w4) && y < (l3 ? h3 : h4)) {
+        const bool l3 = (r & 1) != 0;
+        const bool l4 = (r & 3) == 0 && (c & 1) == 0;
+        const int x = l3 ? tile_x * 8 + c : tile_x * 4 + (c >> 1);
+        const int y = l3 ? tile_y * 4 + (r >> 1) : tile_y * 2 + (r >> 2);
+        if ((l3 || l4) && x < (l3 ? w3 : w4) && y < (l3 ? h3 : h4)) {
             const uint32_t o = __umul24((uint32_t)y, (uint32_t)(l3 ? p3 : p4)) + (uint32_t)x;
 #pragma unroll
-            for (int c = 0; c < 3; c++) emit(c, (l3 ? o3[c] : o4[c]) + o, kFieldL3 + c);
+            for (int ch = 0; ch < 3; ch++) emit(ch, (l3 ? o3[ch] : o4[ch]) + o, kFieldL3 + ch);
         }
     }
 }
@@ -1095,19 +1114,20 @@ VSHIP_INLINE void frontend_tile(const FrontJob& job, const FrontColor* __restric
         float P[3][kBH][kBW];
         compute_tile(job, 0, colors[job.color[0]], geo, tile_x, tile_y, work, l1_area, a, P);
 #pragma unroll
-        for (int f = 0; f < kFields; f++) stash[f * kFrontThreads + t] = field(a, f);
+        for (int f = 0; f < kFields; f++)
+            if (holds_field(f, t)) stash[stash_slot(f, t)] = field(a, f);
         compute_tile(job, 1, colors[job.color[1]], geo, tile_x, tile_y, work, l1_area + kL1Floats, b, P);
         void* o0 = job.out[0];
         void* o1 = job.out[1];
         store_tile(
             job, geo, tile_x, tile_y,
             [&](int plane, uint32_t off, int f) {
-                const float va = stash[f * kFrontThreads + t];
+                const float va = stash[stash_slot(f, t)];
                 gstore1<StoreT>(o0, off, va - kHalfOffset[plane]);
                 gstore1<StoreT>(o1, off, field(b, f) - va);
             },
             [&](int plane, uint32_t off, int f) {
-                const float va0 = stash[f * kFrontThreads + t], va1 = stash[(f + 1) * kFrontThreads + t];
+                const float va0 = stash[stash_slot(f, t)], va1 = stash[stash_slot(f + 1, t)];
                 gstore2<StoreT>(o0, off, va0 - kHalfOffset[plane], va1 - kHalfOffset[plane]);
                 gstore2<StoreT>(o1, off, field(b, f) - va0, field(b, f + 1) - va1);
             });
@@ -1206,19 +1226,33 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
             TileXyb a, b;
             const int next = item + step;
             if constexpr (PHASE == 1) {
-                // image 0's FP32 values, stored by the job that computed it,
-                // in flight (registers) while image 1 is computed
+                // image 0's FP32 values, stored by the job that computed it:
+                // those of levels 0 and 1 in flight (registers) while image 1
+                // is computed, those of levels 2 to 4 (wave 0) loaded after
                 const float* e0 = job.exact0;
                 float ev[kFields];
                 store_tile(
-                    job, geo, tile_x, tile_y, [&](int, uint32_t off, int f) { ev[f] = gload1(e0, off); },
-                    [&](int, uint32_t off, int f) { gload2(e0, off, ev[f], ev[f + 1]); });
+                    job, geo, tile_x, tile_y,
+                    [&](int, uint32_t off, int f) {
+                        if (f < kFieldX2) ev[f] = gload1(e0, off);
+                    },
+                    [&](int, uint32_t off, int f) {
+                        if (f < kFieldX2) gload2(e0, off, ev[f], ev[f + 1]);
+                    });
                 unpack_raw(c1, raw, work, P);
                 if (next < end) {
                     const FrontJob& nj = jobs[front_job_of(jobs, njobs, next)];
                     raw = first_raw(nj, next - nj.tile_start);
                 }
                 compute_tile<SPEC>(job, 1, c1, geo, tile_x, tile_y, work, l1_area + kL1Floats, b, P, true);
+                store_tile(
+                    job, geo, tile_x, tile_y,
+                    [&](int, uint32_t off, int f) {
+                        if (f >= kFieldX2) ev[f] = gload1(e0, off);
+                    },
+                    [&](int, uint32_t off, int f) {
+                        if (f >= kFieldX2) gload2(e0, off, ev[f], ev[f + 1]);
+                    });
                 void* o1 = job.out[1];
                 store_tile(
                     job, geo, tile_x, tile_y,
@@ -1232,7 +1266,8 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
                 raw = raw_tile<kO32>(job, 1, c1, tile_x, tile_y);  // in flight during image 0
                 compute_tile<SPEC>(job, 0, c0, geo, tile_x, tile_y, work, l1_area, a, P, true);
 #pragma unroll
-                for (int f = 0; f < kFields; f++) stash[f * kFrontThreads + t] = field(a, f);
+                for (int f = 0; f < kFields; f++)
+                    if (holds_field(f, t)) stash[stash_slot(f, t)] = field(a, f);
                 unpack_raw(c1, raw, work, P);
                 int nxt_color = 0;
                 if (next < end) {  // the next tile's first image, in flight during image 1
@@ -1249,13 +1284,13 @@ void frontend_kernel(const FrontJob* __restrict__ jobs, const int* __restrict__ 
                 store_tile(
                     job, geo, tile_x, tile_y,
                     [&](int plane, uint32_t off, int f) {
-                        const float va = stash[f * kFrontThreads + t];
+                        const float va = stash[stash_slot(f, t)];
                         gstore1<StoreT>(o0, off, va - kHalfOffset[plane]);
                         gstore1<StoreT>(o1, off, field(b, f) - va);
                         if (e0) gstore1<float>(e0, off, va);
                     },
                     [&](int plane, uint32_t off, int f) {
-                        const float va0 = stash[f * kFrontThreads + t], va1 = stash[(f + 1) * kFrontThreads + t];
+                        const float va0 = stash[stash_slot(f, t)], va1 = stash[stash_slot(f + 1, t)];
                         gstore2<StoreT>(o0, off, va0 - kHalfOffset[plane], va1 - kHalfOffset[plane]);
                         gstore2<StoreT>(o1, off, field(b, f) - va0, field(b, f + 1) - va1);
                         if (e0) gstore2<float>(e0, off, va0, va1);
