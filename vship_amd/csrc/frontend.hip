@@ -467,11 +467,12 @@ VSHIP_INLINE void chroma_420_left(const FrontColor& fc, const float* s_chroma, f
     const int m = t & 31;
     // staged row e = p * crows + r of plane p is row e of both the staged
     // planes and s_mid (planes back to back), so item it of the thread reads
-    // and writes at fixed offsets from its first one (a rolled loop: unrolled,
-    // the FP16 instantiation spills a register at 6 waves per SIMD)
+    // and writes at fixed offsets from its first one (unrolled: the offsets
+    // are the LDS instructions' immediates, and the last item's condition is
+    // uniform per wave)
     const float* src0 = s_chroma + (t >> 5) * ccols + m;
     float* dst0 = s_mid + (t >> 5) * kMidPitch + 2 * m;
-#pragma unroll 1
+#pragma unroll
     for (int it = 0; it < kIters; it++) {
         if ((t >> 5) + kRowStep * it >= 2 * crows) break;
         const float* src = src0 + kRowStep * it * ccols;
