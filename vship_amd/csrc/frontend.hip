@@ -29,7 +29,7 @@ namespace vship {
 
 namespace {
 
-VSHIP_INLINE int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+VSHIP_INLINE int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }  // lo <= hi
 
 // Thread block of level-0 pixels: kBW columns x kBH rows per thread.
 constexpr int kBW = 2, kBH = 2;
@@ -752,10 +752,17 @@ VSHIP_INLINE float lane_ror8(float v) { return __int_as_float(__builtin_amdgcn_m
 VSHIP_INLINE float lane_xor16(float v) { return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F)); }
 
 // P holds the image's unpacked plane values when preloaded (see to_linear).
-template <int SPEC = kFrontGeneric>
+// after_level0(res) runs once the level-0 values are in res (the FP32 layouts
+// issue their stores there, so that they drain while levels 1 to 4 are
+// computed).
+struct NoLevel0Hook {
+    VSHIP_INLINE void operator()(const TileXyb&) const {}
+};
+
+template <int SPEC = kFrontGeneric, typename AfterLevel0 = NoLevel0Hook>
 VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc, const PairGeom& geo, int tile_x,
                                int tile_y, float* lds, float* lds_l1, TileXyb& res, float P[3][kBH][kBW],
-                               bool preloaded = false)
+                               bool preloaded = false, AfterLevel0 after_level0 = {})
 {
     const int t = threadIdx.x;
     const int tx = t % kTX, ty = t / kTX;
@@ -769,6 +776,7 @@ VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc,
     for (int j = 0; j < kBH; j++)
 #pragma unroll
         for (int i = 0; i < kBW; i++) res.y0[j][i] = to_xyb(P[0][j][i], P[1][j][i], P[2][j][i], false).y;
+    after_level0(res);
 
     // ---- level 1 (kBH/2 rows of one column per thread), edge clamp min(2x+1, w-1);
     // a tile inside the image needs no clamp (the same sums)
@@ -896,8 +904,8 @@ VSHIP_INLINE void compute_tile(const FrontJob& job, int k, const FrontColor& fc,
 // pixel pairs: plane offsets, row pitches and their first columns are even,
 // so the pair is aligned).
 template <typename Emit, typename Emit2>
-VSHIP_INLINE void store_tile(const FrontJob& job, const PairGeom& geo, int tile_x, int tile_y, Emit emit,
-                             Emit2 emit2)
+VSHIP_INLINE void store_level0(const FrontJob& job, const PairGeom& geo, int tile_x, int tile_y, Emit emit,
+                               Emit2 emit2)
 {
     const int t = threadIdx.x;
     const int tx = t % kTX, ty = t / kTX;
@@ -913,6 +921,35 @@ VSHIP_INLINE void store_tile(const FrontJob& job, const PairGeom& geo, int tile_
                 emit2(1, o, kFieldY0 + j * kBW);
             } else if (x0 < W) {
                 emit(1, o, kFieldY0 + j * kBW);
+            }
+        }
+    }
+}
+
+// kLevel0 false: levels 1 to 4 only (level 0 went out through
+// store_level0).
+template <bool kLevel0 = true, typename Emit, typename Emit2>
+VSHIP_INLINE void store_tile(const FrontJob& job, const PairGeom& geo, int tile_x, int tile_y, Emit emit,
+                             Emit2 emit2)
+{
+    const int t = threadIdx.x;
+    const int tx = t % kTX, ty = t / kTX;
+    if constexpr (kLevel0) {
+        // store_level0's rows, written out: calling it from here moves the
+        // FP16 instantiations' instruction order enough to spill registers
+        const int W = job.width, H = job.height;
+        const int x0 = tile_x * kFrontTileW + kBW * tx;
+        const int y0 = tile_y * kFrontTileH + kBH * ty;
+#pragma unroll
+        for (int j = 0; j < kBH; j++) {
+            if (y0 + j < H) {
+                const uint32_t o =
+                    (uint32_t)geo.off[0][1] + __umul24((uint32_t)(y0 + j), (uint32_t)geo.pitch[0]) + (uint32_t)x0;
+                if (x0 + 1 < W) {
+                    emit2(1, o, kFieldY0 + j * kBW);
+                } else if (x0 < W) {
+                    emit(1, o, kFieldY0 + j * kBW);
+                }
             }
         }
     }
@@ -1057,20 +1094,35 @@ VSHIP_INLINE void frontend_tile(const FrontJob& job, const FrontColor* __restric
     float* work = lds + (kHalf ? kFrontStashFloats : 0);
     float* l1_area = work + kChromaLdsMax;  // [2 images][kL1Floats]
     const int t = threadIdx.x;
+    // FP32 layout: an image's level-0 plane is stored as soon as it is
+    // computed (the largest part of its stores, which then drains while
+    // levels 1 to 4 are computed), the other levels at the end
+    auto level0_to = [&](void* out) {
+        return [&, out](const TileXyb& r) {
+            store_level0(
+                job, geo, tile_x, tile_y, [&](int, uint32_t off, int f) { gstore1<StoreT>(out, off, field(r, f)); },
+                [&](int, uint32_t off, int f) { gstore2<StoreT>(out, off, field(r, f), field(r, f + 1)); });
+        };
+    };
+    auto levels1to4_to = [&](void* out, const TileXyb& r) {
+        store_tile<false>(
+            job, geo, tile_x, tile_y, [&](int, uint32_t off, int f) { gstore1<StoreT>(out, off, field(r, f)); },
+            [&](int, uint32_t off, int f) { gstore2<StoreT>(out, off, field(r, f), field(r, f + 1)); });
+    };
     if constexpr (!kHalf) {
         if (job.shared) {  // image 0 is stored already: image 1 only
             TileXyb r;
             float P[3][kBH][kBW];
+            void* o1 = job.out[1];
             if constexpr (SPEC == kFrontGeneric) {
-                compute_tile(job, 1, colors[job.color[1]], geo, tile_x, tile_y, work, l1_area + kL1Floats, r, P);
+                compute_tile(job, 1, colors[job.color[1]], geo, tile_x, tile_y, work, l1_area + kL1Floats, r, P,
+                             false, level0_to(o1));
             } else {
                 const FrontColor c1 = front_color<SPEC>(colors[job.color[1]]);
-                compute_tile<SPEC>(job, 1, c1, geo, tile_x, tile_y, work, l1_area + kL1Floats, r, P);
+                compute_tile<SPEC>(job, 1, c1, geo, tile_x, tile_y, work, l1_area + kL1Floats, r, P, false,
+                                   level0_to(o1));
             }
-            void* o1 = job.out[1];
-            store_tile(
-                job, geo, tile_x, tile_y, [&](int, uint32_t off, int f) { gstore1<StoreT>(o1, off, field(r, f)); },
-                [&](int, uint32_t off, int f) { gstore2<StoreT>(o1, off, field(r, f), field(r, f + 1)); });
+            levels1to4_to(o1, r);
             return;
         }
     }
@@ -1079,11 +1131,10 @@ VSHIP_INLINE void frontend_tile(const FrontJob& job, const FrontColor* __restric
         for (int k = 0; k < 2; k++) {
             TileXyb r;
             float P[3][kBH][kBW];
-            compute_tile(job, k, colors[job.color[k]], geo, tile_x, tile_y, work, l1_area + k * kL1Floats, r, P);
             void* out = job.out[k];
-            store_tile(
-                job, geo, tile_x, tile_y, [&](int, uint32_t off, int f) { gstore1<StoreT>(out, off, field(r, f)); },
-                [&](int, uint32_t off, int f) { gstore2<StoreT>(out, off, field(r, f), field(r, f + 1)); });
+            compute_tile(job, k, colors[job.color[k]], geo, tile_x, tile_y, work, l1_area + k * kL1Floats, r, P,
+                         false, level0_to(out));
+            levels1to4_to(out, r);
         }
     } else if constexpr (!kHalf) {
         // layout instantiation: image 1's raw samples load while image 0 is
@@ -1096,20 +1147,16 @@ VSHIP_INLINE void frontend_tile(const FrontJob& job, const FrontColor* __restric
         RawTile raw = raw_tile<true>(job, 0, c0, tile_x, tile_y);
         unpack_raw(c0, raw, work, P);
         raw = raw_tile<true>(job, 1, c1, tile_x, tile_y);
-        compute_tile<SPEC>(job, 0, c0, geo, tile_x, tile_y, work, l1_area, r, P, true);
-        // image 1's samples are unpacked before image 0's stores are issued:
-        // loads and stores share one in-order wait counter, so waiting for the
-        // samples after the stores would wait for the stores to complete
-        unpack_raw(c1, raw, work, P);
         void* o0 = job.out[0];
-        store_tile(
-            job, geo, tile_x, tile_y, [&](int, uint32_t off, int f) { gstore1<StoreT>(o0, off, field(r, f)); },
-            [&](int, uint32_t off, int f) { gstore2<StoreT>(o0, off, field(r, f), field(r, f + 1)); });
-        compute_tile<SPEC>(job, 1, c1, geo, tile_x, tile_y, work, l1_area + kL1Floats, r, P, true);
         void* o1 = job.out[1];
-        store_tile(
-            job, geo, tile_x, tile_y, [&](int, uint32_t off, int f) { gstore1<StoreT>(o1, off, field(r, f)); },
-            [&](int, uint32_t off, int f) { gstore2<StoreT>(o1, off, field(r, f), field(r, f + 1)); });
+        // (image 1's samples were loaded before image 0's level-0 stores, so
+        // waiting for them does not wait for the stores: loads and stores
+        // share one in-order wait counter)
+        compute_tile<SPEC>(job, 0, c0, geo, tile_x, tile_y, work, l1_area, r, P, true, level0_to(o0));
+        unpack_raw(c1, raw, work, P);
+        levels1to4_to(o0, r);
+        compute_tile<SPEC>(job, 1, c1, geo, tile_x, tile_y, work, l1_area + kL1Floats, r, P, true, level0_to(o1));
+        levels1to4_to(o1, r);
     } else {
         TileXyb a, b;
         float P[3][kBH][kBW];
