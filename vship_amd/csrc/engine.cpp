@@ -4,7 +4,9 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <mutex>
 
 #include "errors.hpp"
 #include "kernels.hpp"
@@ -402,6 +404,7 @@ Ssimu2Handler::Ssimu2Handler(const Vship_Colorspace_t& src, const Vship_Colorspa
 
     VSHIP_HIP_CHECK(hipSetDevice(device_));
     VSHIP_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    VSHIP_HIP_CHECK(hipEventCreateWithFlags(&gather_done_, hipEventDisableTiming));
     auto dmalloc = [](void** p, size_t bytes) {
         if (hipMalloc(p, bytes) != hipSuccess) VSHIP_THROW(Vship_OutOfVRAM, "hipMalloc failed in SSIMU2 init");
     };
@@ -487,6 +490,7 @@ Ssimu2Handler::~Ssimu2Handler()
 {
     (void)hipSetDevice(device_);
     if (stream_) (void)hipStreamSynchronize(stream_);
+    if (gather_done_) (void)hipEventDestroy(gather_done_);
     for (int i = 0; i < 2; i++) {
         (void)hipFree(xyb_[i]);
         (void)hipFree(lin_[i]);
@@ -613,6 +617,13 @@ void Ssimu2Handler::launch_all()
 void Ssimu2Handler::run(const uint8_t* const p1[3], const uint8_t* const p2[3], const int64_t ls1[3],
                         const int64_t ls2[3])
 {
+    enqueue(p1, p2, ls1, ls2);
+    complete();
+}
+
+void Ssimu2Handler::enqueue(const uint8_t* const p1[3], const uint8_t* const p2[3], const int64_t ls1[3],
+                            const int64_t ls2[3])
+{
     const uint8_t* const* planes[2] = {p1, p2};
     const int64_t* strides[2] = {ls1, ls2};
     for (int i = 0; i < 2; i++) {
@@ -626,12 +637,52 @@ void Ssimu2Handler::run(const uint8_t* const p1[3], const uint8_t* const p2[3], 
         }
     }
     launch_all();
+}
+
+void Ssimu2Handler::complete()
+{
     VSHIP_HIP_CHECK(hipStreamSynchronize(stream_));
     *h_score_ = score_from_partials(layout_.maps.data(), layout_.maps.count, h_partials_, h_features_);
     have_run_ = true;
 }
 
 namespace {
+
+// Host-plane reads in flight per device: a call holds a slot from its gather
+// launch until the gather has completed (its later kernels run without it).
+// More concurrent readers of pinned host memory share the one PCIe link and
+// lower its rate (8 handlers read 2-5% slower than 4), so handlers beyond
+// kMaxConcurrentGathers wait for a slot before reading.
+constexpr int kMaxConcurrentGathers = 4;
+constexpr int kGateDevices = 64;
+std::mutex g_gather_mutex;
+std::condition_variable g_gather_cv;
+int g_gathers[kGateDevices] = {};
+
+class GatherSlot {
+public:
+    explicit GatherSlot(int device) : device_(device >= 0 && device < kGateDevices ? device : -1)
+    {
+        if (device_ < 0) return;
+        std::unique_lock<std::mutex> lock(g_gather_mutex);
+        g_gather_cv.wait(lock, [&] { return g_gathers[device_] < kMaxConcurrentGathers; });
+        g_gathers[device_]++;
+    }
+    ~GatherSlot()
+    {
+        if (device_ < 0) return;
+        {
+            std::lock_guard<std::mutex> lock(g_gather_mutex);
+            g_gathers[device_]--;
+        }
+        g_gather_cv.notify_all();
+    }
+    GatherSlot(const GatherSlot&) = delete;
+    GatherSlot& operator=(const GatherSlot&) = delete;
+
+private:
+    int device_;
+};
 
 // Device address of host bytes [p, p + bytes) when they lie in pinned host
 // memory the device can read (both ends registered, one contiguous mapping,
@@ -691,14 +742,19 @@ double Ssimu2Handler::compute_host(const uint8_t* const src1[3], const uint8_t* 
         }
     }
     if (direct) {
+        GatherSlot slot(device_);
         VSHIP_HIP_CHECK(launch_gather(g, stream_));
+        VSHIP_HIP_CHECK(hipEventRecord(gather_done_, stream_));
+        enqueue(dev[0], dev[1], ls1, ls2);
+        VSHIP_HIP_CHECK(hipEventSynchronize(gather_done_));
     } else {
         for (int i = 0; i < 2; i++)
             for (int p = 0; p < 3; p++)
                 VSHIP_HIP_CHECK(
                     hipMemcpyAsync(staging_[i][p], srcs[i][p], bytes[i][p], hipMemcpyHostToDevice, stream_));
+        enqueue(dev[0], dev[1], ls1, ls2);
     }
-    run(dev[0], dev[1], ls1, ls2);
+    complete();
     return *h_score_;
 }
 

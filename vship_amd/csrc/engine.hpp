@@ -104,12 +104,18 @@ public:
     void work(double* bytes, double* flops) const;
 
 private:
+    // run = enqueue (the call's plane pointers into the job, every kernel on
+    // the stream) + complete (wait, then the host's FP64 score)
     void run(const uint8_t* const p1[3], const uint8_t* const p2[3], const int64_t ls1[3], const int64_t ls2[3]);
+    void enqueue(const uint8_t* const p1[3], const uint8_t* const p2[3], const int64_t ls1[3],
+                 const int64_t ls2[3]);
+    void complete();
     void launch_all();
     void ensure_staging(int image, const int64_t ls[3]);
 
     int device_;
     hipStream_t stream_ = nullptr;
+    hipEvent_t gather_done_ = nullptr;        // end of a call's host-plane read
     ImageSpec spec_[2];
     PairLayout layout_;
     BlurTaps taps_;
