@@ -4,7 +4,11 @@
 //  - colorspace_key ignores the 4 padding bytes of Vship_Colorspace_t and
 //    separates descriptors that differ in any named field;
 //  - HandleTable ids: reuse after remove, stale ids fail, a reference held by
-//    a call outlives remove().
+//    a call outlives remove();
+//  - the front-end's tile division: (n * magic) >> shift equals n / d for
+//    every divisor up to 16384 (tiles of images up to 2^20 columns) on the
+//    numerators where a wrong reciprocal fails first (multiples of d and
+//    their neighbours, up to 2^31 - 1) and on random ones.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -12,6 +16,7 @@
 
 #include "../../vship_amd/csrc/colorspace_key.hpp"
 #include "../../vship_amd/csrc/handle_table.hpp"
+#include "../../vship_amd/csrc/tile_div.hpp"
 
 #define CHECK(c)                                                          \
     do {                                                                  \
@@ -113,10 +118,38 @@ static void test_handle_table()
     CHECK(Counted::alive == 0);
 }
 
+static void test_tile_div()
+{
+    uint64_t rng = 0x9e3779b97f4a7c15ull;
+    auto next = [&] {
+        rng ^= rng << 13;
+        rng ^= rng >> 7;
+        rng ^= rng << 17;
+        return rng;
+    };
+    const uint32_t top = 0x7fffffffu;
+    for (int d = 1; d <= 16384; d++) {
+        uint32_t magic;
+        int32_t shift;
+        vship::tile_div_magic(d, &magic, &shift);
+        auto ok = [&](uint32_t n) { return vship::tile_div_apply(n, magic, shift) == n / (uint32_t)d; };
+        CHECK(ok(0) && ok(1) && ok(top) && ok(top - 1));
+        // the largest multiples of d below 2^31 and the small ones, +-1
+        const uint32_t kmax = top / (uint32_t)d;
+        for (uint32_t k : {1u, 2u, 3u, kmax - 2, kmax - 1, kmax}) {
+            if (k == 0 || k > kmax) continue;
+            const uint32_t m = k * (uint32_t)d;
+            CHECK(ok(m) && ok(m - 1) && (m == top || ok(m + 1)));
+        }
+        for (int i = 0; i < 64; i++) CHECK(ok((uint32_t)(next() & top)));
+    }
+}
+
 int main()
 {
     test_colorspace_key();
     test_handle_table();
+    test_tile_div();
     std::puts("host units: ok");
     return 0;
 }

@@ -13,6 +13,8 @@
 #include <hip/hip_fp16.h>
 #include <stdint.h>
 
+#include "tile_div.hpp"
+
 namespace vship {
 
 constexpr int kScales = 6;        // pyramid levels of the reference (main.hpp:18)
@@ -132,18 +134,12 @@ struct FrontJob {
     float* exact0;
 };
 
-// Sets a job's tiles_x and the round-up reciprocal that divides by it:
-// magic = ceil(2^(31 + l) / tiles_x) with 2^(l-1) < tiles_x <= 2^l lies below
-// 2^32 and exceeds 2^(31 + l) / tiles_x by less than 2^l / tiles_x, so
-// (local * magic) >> (31 + l) is exactly local / tiles_x for every local below
-// 2^31.
+// Sets a job's tiles_x and the round-up reciprocal that divides by it
+// (tile_div.hpp).
 inline void set_front_tiles(FrontJob& j, int tiles_x)
 {
-    int l = 0;
-    while ((1u << l) < (uint32_t)tiles_x) l++;
     j.tiles_x = tiles_x;
-    j.tiles_x_magic = (uint32_t)((((uint64_t)1 << (31 + l)) + (uint64_t)tiles_x - 1) / (uint64_t)tiles_x);
-    j.tiles_x_shift = 31 + l;
+    tile_div_magic(tiles_x, &j.tiles_x_magic, &j.tiles_x_shift);
 }
 
 // Tile (x, y) of tile `local` of a job, from the job's reciprocal: with the
