@@ -216,6 +216,81 @@ def test_pinned_host_planes_equal_device_planes():
     assert pageable == want
 
 
+def test_pinned_host_planes_from_eight_threads():
+    """Eight handlers on eight threads call Vship_ComputeSSIMU2 on pinned host
+    planes at once, more than the four calls per GPU that may read host planes
+    together: the others wait for a slot, and every call scores exactly as the
+    device-plane entry for its pair."""
+    import ctypes
+    import threading
+    lib = S.load_library()
+    cs = S.colorspace(320, 180)
+    pairs = [cases.perturbed_pair(cs, seed=40 + i, pad=0) for i in range(4)]
+    pinned, bufs = [], []
+
+    def pinned_planes(img):
+        out = []
+        for p in img:
+            ptr = ctypes.c_void_p()
+            S.check(lib.Vship_PinnedMalloc(ctypes.byref(ptr), p.nbytes))
+            pinned.append(ptr)
+            ctypes.memmove(ptr.value, p.ctypes.data, p.nbytes)
+            out.append(ptr.value)
+        return out
+
+    try:
+        want = []
+        with S.SSIMU2Handler(cs, cs) as hd:
+            for ref, dist, st in pairs:
+                dptr = []
+                for img in (ref, dist):
+                    trip = []
+                    for p in img:
+                        b = S.DeviceBuffer(p.nbytes)
+                        b.upload(p)
+                        bufs.append(b)
+                        trip.append(b.address)
+                    dptr.append(trip)
+                want.append(hd.compute_device(dptr[0], dptr[1], st, st))
+        args = []
+        for ref, dist, st in pairs:
+            p1, l1 = S._plane_args(pinned_planes(ref), st)
+            p2, l2 = S._plane_args(pinned_planes(dist), st)
+            args.append((p1, p2, l1, l2))
+        handlers = [S.SSIMU2Handler(cs, cs) for _ in range(8)]
+        got = [[] for _ in range(8)]
+        errors = []
+
+        def worker(k):
+            try:
+                for r in range(6):
+                    i = (k + r) % len(pairs)
+                    p1, p2, l1, l2 = args[i]
+                    sc = ctypes.c_double()
+                    S.check(lib.Vship_ComputeSSIMU2(handlers[k].handle, ctypes.byref(sc), p1, p2, l1, l2))
+                    got[k].append((i, sc.value))
+            except Exception as e:  # reported by the main thread
+                errors.append(e)
+
+        threads = [threading.Thread(target=worker, args=(k,)) for k in range(8)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join(timeout=120)
+        for hd in handlers:
+            hd.close()
+        assert not any(th.is_alive() for th in threads)
+        assert not errors, errors
+    finally:
+        for b in bufs:
+            b.free()
+        for p in pinned:
+            lib.Vship_PinnedFree(p)
+    _say("eight threads on pinned host planes", want)
+    assert all(len(g) == 6 for g in got)
+    assert all(v == want[i] for g in got for i, v in g)
+
+
 def test_line_sizes_beyond_32bit_sample_offsets():
     """Line sizes of 2^24 bytes leave the 32-bit sample offsets of the
     layout-specific front-end; such calls run the generic front-end, and the
