@@ -1405,14 +1405,7 @@ hipError_t launch_linear_rgb(const FrontJob* job, int tiles, const FrontColor* c
 
 size_t frontend_lds_bytes(const FrontColor& fc)
 {
-    size_t chroma = 0;
-    if (fc.subw || fc.subh) {
-        const size_t ccols = (kFrontTileW >> fc.subw) + 4, crows = (kFrontTileH >> fc.subh) + 4;
-        chroma = 2 * ccols * crows;
-        if (fc.subw && fc.subh) chroma += 2 * crows * (fc.subw <= 1 ? kMidPitch : kFrontTileW);
-    }
-    const size_t levels = 3 * 16 * 33 + 3 * 8 * 17 + 3 * 4 * 9;
-    return sizeof(float) * (chroma > levels ? chroma : levels);
+    return sizeof(float) * (size_t)chroma_lds_floats(fc.subw, fc.subh);
 }
 
 size_t frontend_max_lds_bytes()

@@ -5,8 +5,9 @@
 
 namespace vship {
 
-// LDS bytes the front-end kernel needs for images of this colour description,
-// and the largest over every supported chroma subsampling.
+// LDS bytes of the chroma staging for images of this colour description (the
+// linear-RGB kernel's dynamic LDS; the front-end kernels size theirs), and the
+// largest over every supported chroma subsampling.
 size_t frontend_lds_bytes(const FrontColor& fc);
 size_t frontend_max_lds_bytes();
 
