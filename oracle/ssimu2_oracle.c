@@ -8,7 +8,7 @@
  *
  * Every function cites the reference file:line it restates (paths relative to
  * the reference source tree). All arithmetic is double; where the reference
- * rounds to float the oracle does not, so the HIP path is judged against the
+ * rounds to float the oracle does not, so the HIP path is compared with the
  * exact formula. Parity pins (tests/test_oracle_*.py):
  *   - gauss_taps / blur: Appendix-B tap values, scipy.ndimage.correlate with
  *     zero padding divided by the same correlation of an all-ones image,

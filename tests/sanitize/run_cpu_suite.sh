@@ -14,8 +14,8 @@ RT=$("$LLVM/clang" -print-file-name=libclang_rt.asan-x86_64.so)
 SAN="-fsanitize=address,undefined -fno-omit-frame-pointer -fno-sanitize-recover=undefined"
 rm -rf "$WORK"
 mkdir -p "$WORK"
-tar -C "$SRC" --exclude=./.git --exclude=./_work --exclude=./gpurun_out --exclude='*.so' --exclude='_build' \
-    -cf - . | tar -C "$WORK" -xf -
+# the committed tree (its scripts' outputs and run results stay behind)
+git -C "$SRC" ls-files -z | tar -C "$SRC" --null -T - -cf - | tar -C "$WORK" -xf -
 JOBS=$(nproc)
 make -s -j "$JOBS" -C "$WORK/vship_amd/csrc" \
     EXTRA_HIPFLAGS="-Xarch_host -fsanitize=address -Xarch_host -fsanitize=undefined -Xarch_host -fno-omit-frame-pointer" \
