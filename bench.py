@@ -34,6 +34,7 @@ sys.path.insert(0, ROOT)
 
 from vship_amd import ssimu2 as S  # noqa: E402
 from vship_amd import synth  # noqa: E402
+from vship_amd.profile_workload import upload_mixed  # noqa: E402
 
 W, H, PAIRS = 1920, 1080, 240
 HBM_PEAK_GBS = 8000.0
@@ -123,29 +124,6 @@ def run_plain(handlers, args, scores):
         t.join()
     if errors:
         raise errors[0]
-
-
-def upload_mixed(pairs):
-    """Planes of mixed-size pairs in one device buffer (256-byte aligned planes);
-    returns (buf, colorspaces, ptrs[pair][img][3], strides[pair][3])."""
-    align = lambda n: (n + 255) // 256 * 256
-    total = sum(align(p.nbytes) for *_, ref, dist, _q in pairs for p in ref + dist)
-    buf = S.DeviceBuffer(total)
-    css, ptrs, strides = [], [], []
-    off = 0
-    for w, h, bits, ref, dist, _q in pairs:
-        css.append(S.colorspace(w, h, sample=S.SAMPLE_UINT10 if bits == 10 else S.SAMPLE_UINT8))
-        pair = []
-        for img in (ref, dist):
-            planes = []
-            for p in img:
-                buf.upload(p, off)
-                planes.append(buf.address + off)
-                off += align(p.nbytes)
-            pair.append(planes)
-        ptrs.append(pair)
-        strides.append(S.host_strides(ref))
-    return buf, css, ptrs, strides
 
 
 def make_mixed(npairs: int):

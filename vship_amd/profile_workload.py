@@ -4,6 +4,7 @@ python -m vship_amd.profile_workload plain --pairs 8     # plain FP32 path, 1080
 python -m vship_amd.profile_workload batch --pairs 240   # batched FP16 path, 1080p YUV420
 python -m vship_amd.profile_workload batch --blur-config 5  # same, matrix-core blur
 python -m vship_amd.profile_workload batch --sources 16  # 240 pairs over 16 shared reference images
+python -m vship_amd.profile_workload mixed --pairs 512   # 4K 10-bit / 1080p / 720p / 360p in one batch
 python -m vship_amd.profile_workload copy --mib 256      # plain copy kernel (counter calibration)
 python -m vship_amd.profile_workload pcie --handlers 8 --pairs 16 --reps 15
     # Vship_ComputeSSIMU2 on pinned host planes, one host thread per handler
@@ -42,6 +43,52 @@ def _upload_pairs(n: int, seed: int = 0):
             pair.append(trip)
         ptrs.append(pair)
     return bufs, ptrs, strides
+
+
+def upload_mixed(pairs):
+    """Planes of mixed-size pairs (synth.mixed_batch) in one device buffer
+    (256-byte aligned planes); returns (buf, colorspaces, ptrs[pair][img][3],
+    strides[pair][3])."""
+    align = lambda n: (n + 255) // 256 * 256
+    total = sum(align(p.nbytes) for *_, ref, dist, _q in pairs for p in ref + dist)
+    buf = S.DeviceBuffer(total)
+    css, ptrs, strides = [], [], []
+    off = 0
+    for w, h, bits, ref, dist, _q in pairs:
+        css.append(S.colorspace(w, h, sample=S.SAMPLE_UINT10 if bits == 10 else S.SAMPLE_UINT8))
+        pair = []
+        for img in (ref, dist):
+            planes = []
+            for p in img:
+                buf.upload(p, off)
+                planes.append(buf.address + off)
+                off += align(p.nbytes)
+            pair.append(planes)
+        ptrs.append(pair)
+        strides.append(S.host_strides(ref))
+    return buf, css, ptrs, strides
+
+
+def _mixed(a) -> int:
+    """a.pairs pairs of the mixed sizes (3840x2160 10-bit, 1080p, 720p, 360p
+    8-bit YUV420, split 1:4:8:16, seed 3) in one batched call, a.reps calls."""
+    counts = synth.mixed_counts(a.pairs)
+    pairs = synth.mixed_batch(counts, seed=3, workers=16, processes=True)  # forks: before the first HIP call
+    buf, css, ptrs, strides = upload_mixed(pairs)
+    print(f"mixed: {a.pairs} pairs {counts} (3840x2160 10-bit / 1920x1080 / 1280x720 / 640x360 8-bit), "
+          f"{a.reps} reps", flush=True)
+    storage = S.STORAGE_FP16 if a.storage == "fp16" else S.STORAGE_FP32
+    try:
+        with S.SSIMU2Batch(a.pairs, storage) as b:
+            args = (css, css, [p[0] for p in ptrs], [p[1] for p in ptrs], strides, strides)
+            for _ in range(a.reps):
+                scores = b.compute(*args)
+            fr, bl, sc, gr = b.profile(a.reps)
+            print(f"tuning {b.tuning()}; scores mean {np.mean(scores):.5f}; per batch ms frontend {fr:.4f} "
+                  f"blur {bl:.4f} score {sc:.4f} graph {gr:.4f}", flush=True)
+    finally:
+        buf.free()
+    return 0
 
 
 def _pcie(a) -> int:
@@ -101,7 +148,7 @@ def _pcie(a) -> int:
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["plain", "batch", "copy", "pcie"])
+    ap.add_argument("mode", choices=["plain", "batch", "mixed", "copy", "pcie"])
     ap.add_argument("--handlers", type=int, default=8, help="pcie: handlers (one host thread each)")
     ap.add_argument("--pairs", type=int, default=8)
     ap.add_argument("--reps", type=int, default=3)
@@ -124,6 +171,8 @@ def main(argv=None):
         return 0
     if a.mode == "pcie":
         return _pcie(a)
+    if a.mode == "mixed":
+        return _mixed(a)
     bufs, ptrs, st = _upload_pairs(a.pairs)
     cs = S.colorspace(1920, 1080)
     print(f"{a.mode}: {a.pairs} pairs 1920x1080 yuv420p8, {a.reps} reps", flush=True)
