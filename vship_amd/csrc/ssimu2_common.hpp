@@ -33,7 +33,9 @@ constexpr int kDefaultBatchBlurConfig = 8;
 // Batched grids, in workgroups per CU: more than fit on a CU at once (4 blur,
 // 3 front-end), so the hardware dispatcher, not the static item stride,
 // evens out items of unequal cost (measured on 240 1080p pairs: blur 3.76 ms
-// at 4 per CU, 3.32 at 32; front-end 4.20 ms at 4, 3.55 at 32).
+// at 4 per CU, 3.32 at 32; front-end 4.20 ms at 4, 3.55 at 32). 64 per CU
+// (the tuning maximum) was no faster within run-to-run spread on 240 pairs
+// and 1.5% slower on 8.
 constexpr int kDefaultBatchBlurWaves = 32;
 constexpr int kDefaultBatchFrontWaves = 32;
 constexpr int kFrontTileW = 64;
